@@ -1,0 +1,116 @@
+// Degridding (grid -> samples, the adjoint of gridding) and the Pipe-Menon
+// iterative density compensation.  Neither exists in the reference; both
+// reuse the reference's exact tap decisions (fr_taps.h) so that degridding
+// is the exact adjoint of fr_grid_execute.
+#include "fr_internal.h"
+#include "fr_taps.h"
+
+#include <new>
+
+namespace {
+
+// One thread per (sample, coil): out[s][c] = sum over the sample's taps of
+// LUT weight * grid[c][cell], in the reference's z, y, x tap order.
+__global__ void k_degrid(const float *__restrict__ traj, int64_t M, int dim, int N, float hw, int L,
+                         const float *__restrict__ lut, const float2 *__restrict__ grid, size_t cells, int C,
+                         float2 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * C) return;
+    const int64_t s = i / C;
+    const int c = (int)(i - s * C);
+    fr_box b;
+    fr_sample_box(dim, N, traj, M, s, hw, b);
+    const float2 *g = grid + (size_t)c * cells;
+    float re = 0.f, im = 0.f;
+    fr_box_taps(b, b.s, b.e, hw, L, [&](int x, int y, int z, int ki) {
+        const float w = lut[ki];
+        const float2 v = g[((size_t)z * N + y) * N + x];
+        re = __fadd_rn(re, __fmul_rn(w, v.x));
+        im = __fadd_rn(im, __fmul_rn(w, v.y));
+    });
+    out[s * C + c] = make_float2(re, im);
+}
+
+__global__ void k_fill_unit(float2 *__restrict__ d, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = make_float2(1.f, 0.f);
+}
+
+__global__ void k_fill_f32(float *__restrict__ d, int64_t n, float v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
+// w <- w / Re(G^H G w); samples whose denominator is not positive get 0.
+__global__ void k_pm_update(float *__restrict__ w, const float2 *__restrict__ den, int64_t M)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const float d = den[i].x;
+    w[i] = d > 0.f ? __fdiv_rn(w[i], d) : 0.f;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+fr_status fr_degrid_raw(fr_plan *p, const fr_samples *s, const float2 *grid, int C, float2 *out)
+{
+    fr_ctx *ctx = p->ctx;
+    hipLaunchKernelGGL(k_degrid, dim3(nblk(p->M * C, 256)), dim3(256), 0, ctx->stream, s->traj, p->M, p->dim,
+                       (int)p->N, p->kHW, p->L, p->lut, grid, fr_cells(p->dim, p->N), C, out);
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_degrid_execute(fr_plan *p, const fr_samples *s, const fr_grids *g, void *out_dev)
+{
+    if (!p || !s || !g || !out_dev) return FR_ERR_INVALID;
+    fr_ctx *ctx = p->ctx;
+    FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID, "samples do not match the plan");
+    FR_CHECK(ctx, g->dim == p->dim && g->N == p->N, FR_ERR_INVALID, "grids do not match the plan");
+    return fr_degrid_raw(p, s, g->data, g->C, (float2 *)out_dev);
+}
+
+extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones)
+{
+    if (!p || !s) return FR_ERR_INVALID;
+    fr_ctx *ctx = p->ctx;
+    FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID, "samples do not match the plan");
+    FR_CHECK(ctx, iters >= 0, FR_ERR_INVALID, "iters must be >= 0");
+    const int64_t M = s->M;
+    // single-channel unit data sharing the trajectory; its DCF is the weight
+    fr_samples *u = nullptr;
+    fr_grids *g = nullptr;
+    float2 *den = nullptr;
+    fr_status st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &u);
+    if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &g);
+    if (!st) st = fr_alloc(ctx, (void **)&den, sizeof(float2) * M);
+    if (!st) {
+        hipMemcpyAsync(u->traj, s->traj, sizeof(float) * s->dim * M, hipMemcpyDeviceToDevice, ctx->stream);
+        hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, (float2 *)u->data, M);
+        if (init_ones || !s->has_dcf)
+            hipLaunchKernelGGL(k_fill_f32, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, M, 1.f);
+        else
+            hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+        u->has_traj = u->has_dcf = true;
+        for (int it = 0; it < iters && !st; ++it) {
+            st = fr_launch_grid(p, u, g->data, 0);
+            if (!st) st = fr_degrid_raw(p, u, g->data, 1, den);
+            if (!st) hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, den, M);
+        }
+        if (!st) {
+            hipMemcpyAsync(s->dcf, u->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+            s->has_dcf = true;
+            hipError_t e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) st = fr_fail(ctx, FR_ERR_HIP, "pipe-menon: %s", hipGetErrorString(e));
+        }
+    }
+    hipFree(den);
+    fr_grids_destroy(g);
+    fr_samples_destroy(u);
+    return st;
+}

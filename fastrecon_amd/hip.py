@@ -1,0 +1,386 @@
+"""ctypes binding of libfastrecon_hip.so (C ABI in include/fastrecon_hip.h).
+
+This is plumbing for Python callers (tests, bench): every compute call goes
+through the native library.  There is no Python or CPU fallback: if the
+library is missing, loading it raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfastrecon_hip.so")
+
+FR_SAMPLES_C64, FR_SAMPLES_C16 = 0, 1
+FR_LAYOUT_COIL_MAJOR, FR_LAYOUT_SAMPLE_MAJOR = 0, 1
+
+# every symbol the header declares (checked by the CPU test suite)
+EXPORTS = [
+    "fr_kb_lut", "fr_traj_normalize", "fr_traj_transform",
+    "fr_ctx_create", "fr_ctx_destroy", "fr_last_error", "fr_ctx_synchronize", "fr_ctx_stream",
+    "fr_samples_create", "fr_samples_destroy", "fr_samples_set_traj", "fr_samples_set_dcf",
+    "fr_samples_set_coil", "fr_samples_update", "fr_samples_set_data_device", "fr_samples_set_dcf_device",
+    "fr_samples_set_traj_device", "fr_samples_get_dcf",
+    "fr_plan_create", "fr_plan_destroy", "fr_plan_get_info",
+    "fr_grids_create", "fr_grids_destroy", "fr_grids_download", "fr_grids_upload", "fr_grids_device_ptr",
+    "fr_grid_execute", "fr_degrid_execute",
+    "fr_fft_create", "fr_fft_destroy", "fr_fft_execute",
+    "fr_fftshift", "fr_sos", "fr_deapod_map",
+    "fr_dcf_pipe_menon",
+    "fr_malloc", "fr_free", "fr_memcpy_d2h", "fr_memcpy_h2d",
+]
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class fr_tune(C.Structure):
+    _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
+                ("accumulate", C.c_int)]
+
+
+class fr_plan_info(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("taps", C.c_int64), ("records", C.c_int64), ("streams", C.c_int64),
+                ("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
+                ("tiles", C.c_int), ("lds_bytes", C.c_int), ("plan_bytes", C.c_int64), ("build_ms", C.c_double)]
+
+
+_lib = None
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+
+
+def load():
+    """Load the native library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipError(f"{LIB_PATH} is not built (run fastrecon_amd/build.py); there is no fallback")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "fr_kb_lut": [C.c_float, C.c_float, C.c_int, _vp, _vp],
+        "fr_traj_normalize": [_vp, C.c_int64, C.c_int, C.c_int, C.c_uint, _vp],
+        "fr_traj_transform": [_vp, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_float],
+        "fr_ctx_create": [C.c_int, _pp],
+        "fr_ctx_destroy": [_vp],
+        "fr_ctx_synchronize": [_vp],
+        "fr_samples_create": [_vp, C.c_int, C.c_int64, C.c_int, C.c_int, _pp],
+        "fr_samples_destroy": [_vp],
+        "fr_samples_set_traj": [_vp, _vp, C.c_int],
+        "fr_samples_set_dcf": [_vp, _vp],
+        "fr_samples_set_coil": [_vp, C.c_int, _vp],
+        "fr_samples_update": [_vp, C.c_int, C.c_int64, C.c_int64, _vp],
+        "fr_samples_set_data_device": [_vp, _vp, C.c_int],
+        "fr_samples_set_dcf_device": [_vp, _vp],
+        "fr_samples_set_traj_device": [_vp, _vp, C.c_int],
+        "fr_samples_get_dcf": [_vp, _vp],
+        "fr_plan_create": [_vp, _vp, C.c_uint, C.c_float, _vp, C.c_int, _vp, _pp],
+        "fr_plan_destroy": [_vp],
+        "fr_plan_get_info": [_vp, C.POINTER(fr_plan_info)],
+        "fr_grids_create": [_vp, C.c_int, C.c_uint, C.c_int, _pp],
+        "fr_grids_destroy": [_vp],
+        "fr_grids_download": [_vp, C.c_int, _vp],
+        "fr_grids_upload": [_vp, C.c_int, _vp],
+        "fr_grid_execute": [_vp, _vp, _vp],
+        "fr_degrid_execute": [_vp, _vp, _vp, _vp],
+        "fr_fft_create": [_vp, C.c_int, C.c_uint, C.c_int, C.c_int, _pp],
+        "fr_fft_destroy": [_vp],
+        "fr_fft_execute": [_vp, _vp],
+        "fr_fftshift": [_vp, _vp],
+        "fr_sos": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp],
+        "fr_deapod_map": [_vp, C.c_int, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
+        "fr_dcf_pipe_menon": [_vp, _vp, C.c_int, C.c_int],
+        "fr_malloc": [_vp, C.c_size_t, _pp],
+        "fr_free": [_vp, _vp],
+        "fr_memcpy_d2h": [_vp, _vp, _vp, C.c_size_t],
+        "fr_memcpy_h2d": [_vp, _vp, _vp, C.c_size_t],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.fr_last_error.argtypes = [_vp]
+    L.fr_last_error.restype = C.c_char_p
+    L.fr_ctx_stream.argtypes = [_vp]
+    L.fr_ctx_stream.restype = _vp
+    L.fr_grids_device_ptr.argtypes = [_vp]
+    L.fr_grids_device_ptr.restype = _vp
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+# ----------------------------------------------------------- host helpers
+def kb_lut(kwidth=4.0, og=2.0, L=512):
+    out = np.zeros(L, np.float32)
+    beta = np.zeros(1, np.float32)
+    st = load().fr_kb_lut(kwidth, og, L, _ptr(out), _ptr(beta))
+    if st:
+        raise HipError(f"fr_kb_lut failed ({st})")
+    return out, float(beta[0])
+
+
+def normalize_traj(traj, N):
+    """ReconData::normalizeTraj on a host (M, dim) float32 copy."""
+    t = np.array(traj, np.float32, copy=True, order="C")
+    M, dim = t.shape
+    b = np.zeros(2 * dim, np.float32)
+    st = load().fr_traj_normalize(_ptr(t), M, dim, dim, N, _ptr(b))
+    if st:
+        raise HipError(f"fr_traj_normalize failed ({st})")
+    return t, b
+
+
+# --------------------------------------------------------------- handles
+class Context:
+    def __init__(self, device=0):
+        self.L = load()
+        h = _vp()
+        st = self.L.fr_ctx_create(device, C.byref(h))
+        if st:
+            raise HipError(f"fr_ctx_create({device}) failed with status {st} (is a GPU visible?)")
+        self.h = h
+
+    def check(self, st, what):
+        if st:
+            raise HipError(f"{what} failed ({st}): {self.L.fr_last_error(self.h).decode()}")
+
+    @property
+    def stream(self):
+        return self.L.fr_ctx_stream(self.h)
+
+    def sync(self):
+        self.check(self.L.fr_ctx_synchronize(self.h), "fr_ctx_synchronize")
+
+    def close(self):
+        if self.h:
+            self.L.fr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Samples:
+    def __init__(self, ctx, dim, M, coils, sample_type=FR_SAMPLES_C64):
+        self.ctx, self.dim, self.M, self.C = ctx, dim, int(M), int(coils)
+        h = _vp()
+        ctx.check(ctx.L.fr_samples_create(ctx.h, dim, self.M, self.C, sample_type, C.byref(h)), "fr_samples_create")
+        self.h = h
+
+    def set_traj(self, traj):
+        t = np.ascontiguousarray(traj, np.float32)
+        assert t.shape == (self.M, self.dim)
+        self.ctx.check(self.ctx.L.fr_samples_set_traj(self.h, _ptr(t), self.dim), "fr_samples_set_traj")
+
+    def set_dcf(self, dcf):
+        d = np.ascontiguousarray(dcf, np.float32)
+        assert d.shape == (self.M,)
+        self.ctx.check(self.ctx.L.fr_samples_set_dcf(self.h, _ptr(d)), "fr_samples_set_dcf")
+
+    def get_dcf(self):
+        d = np.zeros(self.M, np.float32)
+        self.ctx.check(self.ctx.L.fr_samples_get_dcf(self.h, _ptr(d)), "fr_samples_get_dcf")
+        return d
+
+    def set_coil(self, c, data):
+        d = np.ascontiguousarray(data, np.complex64)
+        assert d.shape == (self.M,)
+        self.ctx.check(self.ctx.L.fr_samples_set_coil(self.h, c, _ptr(d)), "fr_samples_set_coil")
+
+    def set_coils(self, data):
+        for c in range(self.C):
+            self.set_coil(c, data[c])
+
+    def update(self, c, first, data):
+        d = np.ascontiguousarray(data, np.complex64)
+        self.ctx.check(self.ctx.L.fr_samples_update(self.h, c, first, d.shape[0], _ptr(d)), "fr_samples_update")
+
+    def set_data_device(self, ptr, layout):
+        self.ctx.check(self.ctx.L.fr_samples_set_data_device(self.h, _vp(ptr), layout), "fr_samples_set_data_device")
+
+    def set_dcf_device(self, ptr):
+        self.ctx.check(self.ctx.L.fr_samples_set_dcf_device(self.h, _vp(ptr)), "fr_samples_set_dcf_device")
+
+    def set_traj_device(self, ptr, stride):
+        self.ctx.check(self.ctx.L.fr_samples_set_traj_device(self.h, _vp(ptr), stride), "fr_samples_set_traj_device")
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fr_samples_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    def __init__(self, ctx, samples, N, kwidth, lut, tune=None):
+        self.ctx, self.N = ctx, int(N)
+        lut = np.ascontiguousarray(lut, np.float32)
+        t = None
+        if tune:
+            t = fr_tune(**{k: int(v) for k, v in tune.items()})
+        h = _vp()
+        ctx.check(ctx.L.fr_plan_create(ctx.h, samples.h, self.N, float(kwidth), _ptr(lut), len(lut),
+                                       C.byref(t) if t is not None else None, C.byref(h)), "fr_plan_create")
+        self.h = h
+
+    def info(self):
+        i = fr_plan_info()
+        self.ctx.check(self.ctx.L.fr_plan_get_info(self.h, C.byref(i)), "fr_plan_get_info")
+        return {k: getattr(i, k) for k, _ in fr_plan_info._fields_}
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fr_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grids:
+    def __init__(self, ctx, dim, N, coils):
+        self.ctx, self.dim, self.N, self.C = ctx, dim, int(N), int(coils)
+        self.cells = self.N ** dim
+        h = _vp()
+        ctx.check(ctx.L.fr_grids_create(ctx.h, dim, self.N, self.C, C.byref(h)), "fr_grids_create")
+        self.h = h
+
+    @property
+    def ptr(self):
+        return self.ctx.L.fr_grids_device_ptr(self.h)
+
+    def download(self, c):
+        out = np.zeros(self.cells, np.complex64)
+        self.ctx.check(self.ctx.L.fr_grids_download(self.h, c, _ptr(out)), "fr_grids_download")
+        return out
+
+    def download_all(self):
+        return np.stack([self.download(c) for c in range(self.C)])
+
+    def upload(self, c, data):
+        d = np.ascontiguousarray(data, np.complex64)
+        self.ctx.check(self.ctx.L.fr_grids_upload(self.h, c, _ptr(d)), "fr_grids_upload")
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fr_grids_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FFT:
+    def __init__(self, ctx, dim, N, batch, sign=+1):
+        self.ctx = ctx
+        h = _vp()
+        ctx.check(ctx.L.fr_fft_create(ctx.h, dim, int(N), int(batch), int(sign), C.byref(h)), "fr_fft_create")
+        self.h = h
+
+    def __call__(self, grids):
+        self.ctx.check(self.ctx.L.fr_fft_execute(self.h, grids.h), "fr_fft_execute")
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fr_fft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = _vp()
+        ctx.check(ctx.L.fr_malloc(ctx.h, self.nbytes, C.byref(p)), "fr_malloc")
+        self.ptr = p.value
+
+    def to_host(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else count
+        out = np.zeros(n, dtype)
+        self.ctx.check(self.ctx.L.fr_memcpy_d2h(self.ctx.h, _ptr(out), _vp(self.ptr), out.nbytes), "fr_memcpy_d2h")
+        return out
+
+    def from_host(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.L.fr_memcpy_h2d(self.ctx.h, _vp(self.ptr), _ptr(a), a.nbytes), "fr_memcpy_h2d")
+
+    def close(self):
+        if self.ptr:
+            self.ctx.L.fr_free(self.ctx.h, _vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- calls
+def grid_execute(plan, samples, grids):
+    plan.ctx.check(plan.ctx.L.fr_grid_execute(plan.h, samples.h, grids.h), "fr_grid_execute")
+
+
+def degrid_execute(plan, samples, grids, out_buf):
+    plan.ctx.check(plan.ctx.L.fr_degrid_execute(plan.h, samples.h, grids.h, _vp(out_buf.ptr)), "fr_degrid_execute")
+
+
+def fftshift(ctx, grids):
+    ctx.check(ctx.L.fr_fftshift(ctx.h, grids.h), "fr_fftshift")
+
+
+def sos(ctx, grids, rc, shift=True, deapod=None, out=None):
+    rcx, rcy, rcz = rc
+    if grids.dim == 2:
+        rcz = 1
+    n = rcx * rcy * rcz
+    if out is None:
+        out = DeviceBuffer(ctx, 4 * n)
+    ctx.check(ctx.L.fr_sos(ctx.h, grids.h, rcx, rcy, rcz, 1 if shift else 0,
+                           _vp(deapod.ptr) if deapod is not None else None, _vp(out.ptr)), "fr_sos")
+    return out
+
+
+def deapod_map(ctx, dim, N, kwidth, lut, centre, rc):
+    rcx, rcy, rcz = rc
+    if dim == 2:
+        rcz = 1
+    out = DeviceBuffer(ctx, 4 * rcx * rcy * rcz)
+    lut = np.ascontiguousarray(lut, np.float32)
+    c = np.ascontiguousarray(centre, np.float32)
+    ctx.check(ctx.L.fr_deapod_map(ctx.h, dim, N, float(kwidth), _ptr(lut), len(lut), _ptr(c), rcx, rcy, rcz,
+                                  _vp(out.ptr)), "fr_deapod_map")
+    return out
+
+
+def pipe_menon(plan, samples, iters=10, init_ones=True):
+    plan.ctx.check(plan.ctx.L.fr_dcf_pipe_menon(plan.h, samples.h, iters, 1 if init_ones else 0),
+                   "fr_dcf_pipe_menon")

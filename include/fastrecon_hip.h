@@ -1,0 +1,188 @@
+/*
+ * fastrecon_hip.h -- C ABI of libfastrecon_hip.so, the MI355X (gfx950)
+ * backend behind the fastrecon C++ API (ReconData / GridLut / ImageFilter /
+ * ImageData).
+ *
+ * The reference (pengwg/fastrecon) has no C ABI: its GPU backend is a set of
+ * C++ subclasses (cuReconData, cuGridLut, cuImageData, cuImageFilter) that
+ * the factories return.  Each entry point below replaces one piece of that
+ * backend; the reference interface it stands in for is cited on each.
+ *
+ * Conventions
+ *   - Every call returns fr_status (FR_OK == 0) and never exits the process;
+ *     fr_last_error() gives a message for the last failure on a context.
+ *   - Handles are opaque and own their device memory.
+ *   - All device work of a context runs in order on the context's one HIP
+ *     stream (default priority).  Calls that read or write host memory
+ *     synchronise that stream before returning, so host pointers are only
+ *     borrowed for the duration of the call.  Calls taking device pointers
+ *     are asynchronous on the context stream.
+ *   - One context per host thread.
+ *   - Results are deterministic: the gridding kernels use no global atomics
+ *     and accumulate every grid cell in ascending sample order, the order of
+ *     the reference CPU loop (ReconData/GridLut.cpp:96-145).
+ */
+#ifndef FASTRECON_HIP_H
+#define FASTRECON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FR_OK = 0,
+    FR_ERR_INVALID = 1,      /* bad argument / size mismatch                */
+    FR_ERR_HIP = 2,          /* HIP runtime error                           */
+    FR_ERR_NOMEM = 3,        /* device allocation failed                    */
+    FR_ERR_STATE = 4,        /* call out of order (e.g. execute before plan)*/
+    FR_ERR_UNSUPPORTED = 5,  /* configuration this build does not handle   */
+    FR_ERR_LIBRARY = 6       /* rocFFT or another library failed            */
+} fr_status;
+
+typedef enum { FR_SAMPLES_C64 = 0, FR_SAMPLES_C16 = 1 } fr_sample_type;
+typedef enum { FR_LAYOUT_COIL_MAJOR = 0, FR_LAYOUT_SAMPLE_MAJOR = 1 } fr_layout;
+
+typedef struct fr_ctx fr_ctx;
+typedef struct fr_samples fr_samples;
+typedef struct fr_plan fr_plan;
+typedef struct fr_grids fr_grids;
+typedef struct fr_fft fr_fft;
+
+/* Tuning knobs for a gridding plan; zero fields mean "choose automatically".
+ * The LUT length is never tuned in parity mode: it is the L passed in. */
+typedef struct {
+    int tile_x, tile_y, tile_z;  /* grid tile owned by one workgroup        */
+    int slabs;                   /* waves per workgroup; each owns a slab    */
+    int accumulate;              /* 0: LDS add instruction, 1: LDS read-add-write */
+} fr_tune;
+
+typedef struct {
+    int64_t samples, taps, records, streams;
+    int tile_x, tile_y, tile_z, slabs, tiles, lds_bytes;
+    int64_t plan_bytes;
+    double build_ms;
+} fr_plan_info;
+
+/* ------------------------------------------------------------ host helpers */
+/* ConvKernel::ConvKernel (ReconData/ConvKernel.cpp:7-30): the L-entry radial
+ * Kaiser-Bessel LUT, normalised so lut[0] == 1; beta (may be NULL) gets the
+ * Beatty et al. shape parameter.  Host-only, no device needed. */
+fr_status fr_kb_lut(float kWidth, float og, int L, float *lut, float *beta);
+/* ReconData::storeTrajComponent bounds (ReconData.cpp:56-81) followed by
+ * normalizeTraj (ReconData.cpp:211-225), in place on a host AoS trajectory:
+ * every component uses component 0's bounds.  bounds (may be NULL) receives
+ * the normalised (min, max) per component, like m_bounds.  Host-only. */
+fr_status fr_traj_normalize(float *traj, int64_t M, int stride, int dim, unsigned N, float *bounds);
+/* transformLocalTraj (ReconData.cpp:201-209): x <- (x + tr) * scale. */
+fr_status fr_traj_transform(float *traj, int64_t M, int stride, int dim, float tr, float scale);
+
+/* ---------------------------------------------------------------- context */
+/* No reference counterpart (the reference uses the CUDA default stream and
+ * checks no errors, cuGridLut.cu:303). */
+fr_status fr_ctx_create(int device, fr_ctx **out);
+fr_status fr_ctx_destroy(fr_ctx *ctx);
+const char *fr_last_error(const fr_ctx *ctx);
+fr_status fr_ctx_synchronize(fr_ctx *ctx);
+void *fr_ctx_stream(fr_ctx *ctx);              /* the hipStream_t */
+
+/* ---------------------------------------------------------------- samples */
+/* Replaces cuReconData (ReconData/cuReconData.h:6-47): device copies of the
+ * trajectory, DCF and all coils (the reference keeps one coil resident,
+ * cuReconData.cu:12-31).  Coordinates must already be normalised to grid
+ * units by ReconData::normalizeTraj (ReconData.cpp:211-225), which the C++
+ * layer does on the host with the reference's exact FP32 arithmetic. */
+fr_status fr_samples_create(fr_ctx *ctx, int dim, int64_t M, int coils,
+                            int sample_type, fr_samples **out);
+fr_status fr_samples_destroy(fr_samples *s);
+/* traj: M samples, `stride` floats apart, `dim` coordinates each (the
+ * reference's Point<float> AoS is stride 4, ReconData.h:12-15). */
+fr_status fr_samples_set_traj(fr_samples *s, const float *traj, int stride);
+/* ReconData::setDcf (ReconData.cpp:45-54) / cuGetDcf (cuReconData.cu:39-50). */
+fr_status fr_samples_set_dcf(fr_samples *s, const float *dcf);
+/* ReconData::addChannelData (ReconData.cpp:33-43) / cuGetChannelData
+ * (cuReconData.cu:12-31): M complex64 values of coil c (converted to FP16
+ * on the device for FR_SAMPLES_C16). */
+fr_status fr_samples_set_coil(fr_samples *s, int coil, const void *data);
+/* ReconData::updateSingleAcquisition (ReconData.cpp:179-189,
+ * cuReconData.cu:52-62): overwrite n samples of coil c starting at first. */
+fr_status fr_samples_update(fr_samples *s, int coil, int64_t first, int64_t n, const void *data);
+/* Device-pointer variants (asynchronous): data in complex64, either coil-
+ * major [C][M] or sample-major [M][C]; dcf and traj in device memory. */
+fr_status fr_samples_set_data_device(fr_samples *s, const void *data, int layout);
+fr_status fr_samples_set_dcf_device(fr_samples *s, const float *dcf);
+fr_status fr_samples_set_traj_device(fr_samples *s, const float *traj, int stride);
+fr_status fr_samples_get_dcf(const fr_samples *s, float *host_dcf);
+
+/* ------------------------------------------------------------------- plan */
+/* Replaces cuGridLut::plan (ReconData/cuGridLut.cu:151-226), the sorted-bin
+ * gridding LUT.  N = unsigned(reconSize * og) (GridLut.cpp:45); lut is the
+ * ConvKernel table of length L (ConvKernel.cpp:7-30); kWidth the full
+ * kernel width.  The plan depends on the trajectory only. */
+fr_status fr_plan_create(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth,
+                         const float *lut, int L, const fr_tune *tune, fr_plan **out);
+fr_status fr_plan_destroy(fr_plan *p);
+fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info);
+
+/* ------------------------------------------------------------------ grids */
+/* Replaces cuImageData (ImageData/cuImageData.h:7-36): C device-resident
+ * grids of N^dim complex64, index x + N*y + N*N*z (GridLut.cpp:111). */
+fr_status fr_grids_create(fr_ctx *ctx, int dim, unsigned N, int coils, fr_grids **out);
+fr_status fr_grids_destroy(fr_grids *g);
+fr_status fr_grids_download(const fr_grids *g, int coil, void *host);
+fr_status fr_grids_upload(fr_grids *g, int coil, const void *host);
+void *fr_grids_device_ptr(fr_grids *g);
+
+/* Replaces cuGridLut::execute / griddingChannel / gridding_kernel
+ * (cuGridLut.cu:28-48, 231-309) and GridLut::execute (GridLut.cpp:53-78):
+ * grids every coil of s into out (all coils in one launch). */
+fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *out);
+
+/* Degridding (grid -> samples), the adjoint of fr_grid_execute with the
+ * same taps, no DCF.  Net-new (the reference has none).  s supplies the
+ * trajectory the plan was built on; out: device complex64, sample-major
+ * [M][g->C]. */
+fr_status fr_degrid_execute(fr_plan *p, const fr_samples *s, const fr_grids *g, void *out_dev);
+
+/* -------------------------------------------------------------------- FFT */
+/* Replaces cuFFT (ImageFilter/cuFFT.cpp:15-37) and FFT (FFT.cpp:30-91):
+ * one batched in-place C2C rocFFT plan over all coils, unnormalised.
+ * sign +1 is FFTW_BACKWARD / CUFFT_INVERSE, the reference default. */
+fr_status fr_fft_create(fr_ctx *ctx, int dim, unsigned N, int batch, int sign, fr_fft **out);
+fr_status fr_fft_destroy(fr_fft *f);
+fr_status fr_fft_execute(fr_fft *f, fr_grids *g);
+
+/* ---------------------------------------------------------- image stage */
+/* ImageFilter::fftShift (ImageFilter.cpp:40-53, 231-280), in place, even N. */
+fr_status fr_fftshift(fr_ctx *ctx, fr_grids *g);
+/* Fused crop + sum-of-squares + magnitude (ImageFilter.cpp:159-209 and
+ * Main.cpp:175-181): out_dev[rcz][rcy][rcx] = sqrt(sum_c |v_c|^2) / deapod,
+ * coils summed in order in FP32.  apply_shift != 0 folds fftShift into the
+ * read (the grids are left unshifted).  deapod_dev may be NULL. */
+fr_status fr_sos(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz,
+                 int apply_shift, const float *deapod_dev, float *out_dev);
+/* Deapodisation map (net-new): |FFT| of one unit sample gridded at the
+ * normalised position `centre` with the same LUT, shifted and cropped.
+ * out_dev: rcx*rcy*rcz floats. */
+fr_status fr_deapod_map(fr_ctx *ctx, int dim, unsigned N, float kWidth, const float *lut, int L,
+                        const float *centre, unsigned rcx, unsigned rcy, unsigned rcz,
+                        float *out_dev);
+
+/* --------------------------------------------------- density compensation */
+/* Pipe-Menon iterative DCF (net-new): w <- w / G^H G w, `iters` times,
+ * starting from the samples' current DCF (or 1 if init_ones), written back
+ * to the samples' DCF.  Uses this plan's gridding and degridding kernels. */
+fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones);
+
+/* --------------------------------------------------------- device memory */
+fr_status fr_malloc(fr_ctx *ctx, size_t bytes, void **dev);
+fr_status fr_free(fr_ctx *ctx, void *dev);
+fr_status fr_memcpy_d2h(fr_ctx *ctx, void *host, const void *dev, size_t bytes);
+fr_status fr_memcpy_h2d(fr_ctx *ctx, void *dev, const void *host, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
