@@ -1,0 +1,197 @@
+"""Benchmark of the gridding hot path on one MI355X.
+
+Workload (the headline configuration): 3D radial kooshball, 51,200 spokes on
+Fibonacci-sphere directions x 256 samples (13,107,200 samples per coil),
+32 coils, complex64 seeded Gaussian data, DCF |k|^2, Kaiser-Bessel width 4,
+oversampling 2 (128^3 image -> 256^3 grid), 512-entry LUT, FP32
+accumulation.  Inputs are resident in HBM before the timed region.
+
+A step is one gridding pass over all 32 coils (one kernel launch).  `value`
+is gridded coil-samples per second = C * M / t_step.  The JSON line also
+carries the roofline of the gridding kernel (algorithmic bytes
+C*M*8 + M*(4*3+4) + C*N^3*8 per launch over its average duration measured
+with HIP events on the launch stream), the full reconstruction latency
+(grid + batched FFT + fused shift/crop/SOS), the plan build time, and a CPU
+baseline: the oracle's FP32 restatement of the reference CPU gridding loop
+timed on this host's cores on a bounded slice of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--spokes", type=int, default=51200)
+    ap.add_argument("--readout", type=int, default=256)
+    ap.add_argument("--coils", type=int, default=32)
+    ap.add_argument("--image", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-recon", action="store_true")
+    ap.add_argument("--accumulate", type=int, default=0)
+    ap.add_argument("--exact", action="store_true", help="exact-order plan for the timed steps")
+    ap.add_argument("--no-exact-check", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(traj_n, dcf, N, coils, lut, target_s):
+    """Oracle FP32 restatement of GridLut::griddingChannel on a bounded
+    slice (the first spokes of the same trajectory), OpenMP over coils with
+    min(coils, host threads) threads as Program/Main.cpp:111."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(coils, cores, 16))
+    readout = 256
+    m = readout * 64
+    rng = np.random.default_rng(12345)
+
+    def data(n):
+        return (rng.standard_normal((coils, n), np.float32)
+                + 1j * rng.standard_normal((coils, n), np.float32)).astype(np.complex64)
+
+    tr = traj_n[:m]
+    d = data(m)
+    t0 = time.perf_counter()
+    oracle.grid_f32(tr, dcf[:m], d, N, 4.0, lut, nthreads=threads)
+    dt = time.perf_counter() - t0
+    # scale the slice to the target time
+    scale = max(1, min(int(target_s / max(dt, 1e-3)), 64))
+    m2 = min(m * scale, traj_n.shape[0])
+    tr = traj_n[:m2]
+    d = data(m2)
+    t0 = time.perf_counter()
+    oracle.grid_f32(tr, dcf[:m2], d, N, 4.0, lut, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": coils * m2 / dt, "unit": "coil-samples/s", "cores": threads, "kind": "port",
+            "sample": f"first {m2 // readout} spokes x {readout} samples x {coils} coils of the same workload "
+                      f"on the full {N}^3 grid, {dt:.2f} s, oracle FP32 restatement of GridLut.cpp:80-147 "
+                      f"(OpenMP over coils)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "design-only)"}))
+        sys.exit(2)
+    import torch
+    from fastrecon_amd import hip
+    from fastrecon_amd import synthetic as syn
+
+    og, kw, L = 2.0, 4.0, 512
+    N = syn.grid_size(a.image, og)
+    C = a.coils
+    traj, dcf = syn.kooshball3d(a.spokes, a.readout)
+    M = traj.shape[0]
+    lut, _ = hip.kb_lut(kw, og, L)
+    traj_n, _ = hip.normalize_traj(traj, N)
+
+    ctx = hip.Context(0)
+    s = hip.Samples(ctx, 3, M, C)
+    s.set_traj(traj_n)
+    s.set_dcf(dcf)
+    # seeded complex Gaussian data generated on the device, sample-major
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(2)
+    data = torch.randn(M * C * 2, generator=gen, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    s.set_data_device(data.data_ptr(), hip.FR_LAYOUT_SAMPLE_MAJOR)
+    ctx.sync()
+    del data
+    torch.cuda.empty_cache()
+
+    t0 = time.perf_counter()
+    plan = hip.Plan(ctx, s, N, kw, lut, {"accumulate": a.accumulate, "exact_order": 1 if a.exact else 0})
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    info = plan.info()
+    grids = hip.Grids(ctx, 3, N, C)
+
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for _ in range(a.warmup):
+        hip.grid_execute(plan, s, grids)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    t_wall = time.perf_counter()
+    for e0, e1 in ev:
+        e0.record(stream)
+        hip.grid_execute(plan, s, grids)
+        e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t_wall) * 1e3 / a.steps
+    kms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    ms = float(np.mean(kms))
+
+    bytes_alg = C * M * 8 + M * (4 * 3 + 4) + C * N ** 3 * 8
+    achieved = bytes_alg / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "gridded samples/s, recon ms (3D radial, 32 coils); learned-DCF vs iterative",
+        "value": C * M / (ms * 1e-3),
+        "unit": "coil-samples/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "wall_ms_per_step": wall_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"kooshball3d: 3D radial {a.spokes} Fibonacci spokes x {a.readout} samples, "
+                               f"{C} coils, {a.image}^3 image, og 2 -> {N}^3 grid, KB width 4, L 512, "
+                               f"complex64, DCF |k|^2",
+                   "samples_per_coil": M, "coils": C, "grid": N,
+                   "tile": [info["tile_x"], info["tile_y"], info["tile_z"]], "slabs": info["slabs"],
+                   "accumulate": "lds read-add-write" if a.accumulate == 0 else "lds float atomic add",
+                   "order": "exact (CPU summation order)" if a.exact else "sliced (fixed-order slice combine)",
+                   "items": info["items"], "heavy_units": info["heavy_units"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": bytes_alg, "kernel": "k_grid_tiles"},
+        "plan_ms": plan_ms, "plan_taps": info["taps"], "plan_bytes": info["plan_bytes"],
+    }
+
+    if not a.no_recon:
+        fft = hip.FFT(ctx, 3, N, C, +1)
+        img = hip.DeviceBuffer(ctx, 4 * a.image ** 3)
+        rc = (a.image, a.image, a.image)
+        hip.grid_execute(plan, s, grids); fft(grids); hip.sos(ctx, grids, rc, True, None, img)
+        ctx.sync()
+        r = []
+        for _ in range(3):
+            e0, e1, e2, e3 = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e0.record(stream)
+            hip.grid_execute(plan, s, grids)
+            e1.record(stream)
+            fft(grids)
+            e2.record(stream)
+            hip.sos(ctx, grids, rc, True, None, img)
+            e3.record(stream)
+            ctx.sync()
+            r.append((e0.elapsed_time(e1), e1.elapsed_time(e2), e2.elapsed_time(e3), e0.elapsed_time(e3)))
+        r = np.median(np.array(r), axis=0)
+        out["recon_ms"] = float(r[3])
+        out["recon_stages_ms"] = {"grid": float(r[0]), "fft": float(r[1]), "shift_crop_sos": float(r[2])}
+        fft.close()
+        img.close()
+
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(traj_n, dcf, N, C, lut, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

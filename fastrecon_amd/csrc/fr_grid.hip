@@ -1,110 +1,271 @@
-// Gridding execute: one workgroup per (grid tile, chunk of <= 32 coils).
+// Gridding execute: one wave per work unit, all coils (in chunks of 32) of
+// a unit accumulated in that wave's slice of LDS.
 //
 // Replaces cuGridLut::griddingChannel / gridding_kernel
 // (ReconData/cuGridLut.cu:231-309) and the CPU loop GridLut::griddingChannel
 // (GridLut.cpp:80-147).
 //
-// Layout and mapping (gfx950, wave64):
-//   * samples are sample-major [M][C] complex, so the 32 coils of a sample
-//     are one contiguous 256-byte read;
-//   * the tile's accumulator lives in LDS as [cell][2*coils (+1 pad)] FP32;
-//   * wave w of the workgroup owns slab w of the tile and walks that slab's
-//     record stream (scalar loads: the stream is wave-uniform); lane p holds
-//     component p (coil p/2, re or im) of the current sample, already scaled
-//     by its DCF exactly like the reference's `dcf * sample`;
-//   * every tap is one multiply w * d and one LDS add into [cell][p]; lanes
-//     address consecutive words, so the add is bank-conflict free.
-// A cell is only ever touched by the wave owning its slab, and that wave
-// walks samples in ascending order, so each cell sums its contributions in
-// the reference CPU order with the same FP32 operations: the result is
-// deterministic, uses no global atomics, and matches the CPU path bit for
-// bit.  The finished tile is written once, coalesced along x.
+// Mapping (gfx950, wave64):
+//   * samples are sample-major [M][C] complex, so the (up to) 32 coils of a
+//     sample are one contiguous 256-byte read: lane p holds component p
+//     (coil p/2, re or im), scaled by the DCF exactly like the reference's
+//     `dcf * sample`;
+//   * the unit's accumulator is [cell][2*coils + 1 pad] FP32 in LDS; a tap is
+//     one multiply w * d and one LDS add into [cell][p] (consecutive words,
+//     bank-conflict free);
+//   * record headers are fetched 64 at a time with one vector load and read
+//     with readlane; taps are fetched 64 at a time into VGPRs one chunk
+//     ahead, staged through a 512-byte LDS buffer and read by broadcast; the
+//     next record's sample data is requested before the current record's
+//     taps are applied.  This keeps several HBM requests in flight per wave.
+// Each cell belongs to exactly one unit, its wave walks the unit's records
+// in ascending sample order, so every cell sums its contributions in the
+// reference CPU order with the same FP32 operations: deterministic, no
+// global atomics, bit-identical to the CPU path.  A unit's cells are written
+// once, coalesced along x.
 #include "fr_internal.h"
+
+#include <algorithm>
 
 namespace {
 
 template <typename TS>
-__device__ __forceinline__ float load_comp(const TS *__restrict__ data, uint32_t s, int C, int coil, int comp);
+__device__ __forceinline__ float load_comp(const TS *__restrict__ data, uint32_t s, int C, int lanecomp);
 
 template <>
-__device__ __forceinline__ float load_comp<float2>(const float2 *__restrict__ data, uint32_t s, int C, int coil,
-                                                   int comp)
+__device__ __forceinline__ float load_comp<float2>(const float2 *__restrict__ data, uint32_t s, int C, int lanecomp)
 {
-    const float *f = reinterpret_cast<const float *>(data);
-    return f[((size_t)s * C + coil) * 2 + comp];
+    return reinterpret_cast<const float *>(data)[(size_t)s * C * 2 + lanecomp];
 }
 
 template <>
-__device__ __forceinline__ float load_comp<__half2>(const __half2 *__restrict__ data, uint32_t s, int C, int coil,
-                                                    int comp)
+__device__ __forceinline__ float load_comp<__half2>(const __half2 *__restrict__ data, uint32_t s, int C, int lanecomp)
 {
-    const __half *h = reinterpret_cast<const __half *>(data);
-    return __half2float(h[((size_t)s * C + coil) * 2 + comp]);
+    return __half2float(reinterpret_cast<const __half *>(data)[(size_t)s * C * 2 + lanecomp]);
+}
+
+__device__ __forceinline__ uint32_t rl(uint32_t v, int lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+// A zero the compiler cannot prove uniform.  Indexing with it turns a load
+// at a wave-uniform address into a vector load: vector loads complete in
+// order and are waited on one at a time (vmcnt), whereas scalar loads may
+// complete out of order, so waiting on any one of them drains them all and
+// every prefetch in flight.
+__device__ __forceinline__ uint32_t opaque_zero()
+{
+    uint32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+constexpr int BATCH = 32;          // records per batch
+constexpr int ACC_STRIDE = FR_ACC_STRIDE;   // floats per accumulator row (64 components + pad)
+constexpr int STAGE = 8;           // 64-tap chunks staged per batch (512 taps)
+
+// One batch of records [base, base + k) with taps from tb; its loads are
+// issued by prep() and consumed by compute() one batch later.
+template <typename TS>
+struct batch_regs {
+    float d[BATCH];                // lane p: component p of each record's sample
+    float w;                       // lane j: DCF of record j
+    uint2 ch[STAGE];               // lane l: tap cb + 64 q + l
+    uint64_t cb, tb;
+    int k;
+};
+
+template <typename TS>
+__device__ __forceinline__ void prep(batch_regs<TS> &B, uint2 &h, uint64_t &nb, uint64_t &ntb, uint64_t re,
+                                     const uint2 *__restrict__ H, const uint2 *__restrict__ T,
+                                     const TS *__restrict__ data, const float *__restrict__ dcf, int C, int lc,
+                                     uint32_t vz, uint32_t *meta, int lane)
+{
+    // h holds the headers of records nb .. nb + 63 (lane l: record nb + l)
+    const uint64_t cb = ntb & ~(uint64_t)63;
+    int k = 0;
+    uint64_t te = ntb;
+    for (int j = 0; j < BATCH; ++j) {
+        if (nb + j >= re) break;
+        const uint32_t n = rl(h.y, j);
+        if (te + n - cb > 64 * STAGE) break;   // records carry at most 64 taps, so k >= 1
+        te += n;
+        ++k;
+    }
+    B.k = k;
+    B.cb = cb;
+    B.tb = ntb;
+    meta[lane] = h.y;                          // tap counts of the batch's records
+#pragma clang loop unroll(full)
+    for (int j = 0; j < BATCH; ++j) {          // unconditional: past-the-end records read a valid row
+        const uint32_t s = rl(h.x, j < k ? j : 0);
+        B.d[j] = load_comp<TS>(data, s, C, lc);
+    }
+    B.w = dcf[h.x + vz];
+#pragma clang loop unroll(full)
+    for (int q = 0; q < STAGE; ++q) B.ch[q] = T[cb + 64 * q + lane];
+    nb += k;
+    ntb = te;
+    h = H[nb + lane];                          // next batch's headers (H is padded)
+}
+
+// Apply the taps of batch B.  A record's taps are distinct cells, so the
+// accumulator words of a group of taps are read together, updated and
+// written back (LDS keeps a wave's accesses in order, so the next group sees
+// these writes).  Tap cells are stored as byte offsets of row cell*ACC_STRIDE.
+template <int ACC, typename TS>
+__device__ __forceinline__ void compute(const batch_regs<TS> &B, const uint32_t *meta, uint2 *stage, char *accb,
+                                        uint32_t lane4)
+{
+#pragma clang loop unroll(full)
+    for (int q = 0; q < STAGE; ++q) stage[64 * q + (lane4 >> 2)] = B.ch[q];
+    uint32_t off = (uint32_t)(B.tb - B.cb);
+#pragma clang loop unroll(full)
+    for (int j = 0; j < BATCH; ++j) {
+        const uint32_t n = __builtin_amdgcn_readfirstlane(j < B.k ? meta[j] : 0u);
+        const float w = __uint_as_float(rl(__float_as_uint(B.w), j));
+        const float d = __fmul_rn(w, B.d[j]);              // dcf * sample (GridLut.cpp:109)
+        const uint2 *tp = stage + off;
+        uint32_t q = 0;
+        if constexpr (ACC == 0) {
+            for (; q + 8 <= n; q += 8) {
+                float *p[8];
+                float v[8], x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint2 a = tp[q + u];
+                    p[u] = reinterpret_cast<float *>(accb + (a.x + lane4));
+                    v[u] = __fmul_rn(__uint_as_float(a.y), d);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = *p[u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) *p[u] = __fadd_rn(x[u], v[u]);
+            }
+            for (; q + 2 <= n; q += 2) {
+                const uint2 a0 = tp[q], a1 = tp[q + 1];
+                float *p0 = reinterpret_cast<float *>(accb + (a0.x + lane4));
+                float *p1 = reinterpret_cast<float *>(accb + (a1.x + lane4));
+                const float x0 = *p0, x1 = *p1;
+                *p0 = __fadd_rn(x0, __fmul_rn(__uint_as_float(a0.y), d));
+                *p1 = __fadd_rn(x1, __fmul_rn(__uint_as_float(a1.y), d));
+            }
+            if (q < n) {
+                const uint2 a0 = tp[q];
+                float *p0 = reinterpret_cast<float *>(accb + (a0.x + lane4));
+                *p0 = __fadd_rn(*p0, __fmul_rn(__uint_as_float(a0.y), d));
+            }
+        } else {
+            for (; q < n; ++q) {                            // LDS float atomic add (slower on gfx950)
+                const uint2 a0 = tp[q];
+                atomicAdd(reinterpret_cast<float *>(accb + (a0.x + lane4)), __fmul_rn(__uint_as_float(a0.y), d));
+            }
+        }
+        off += n;
+    }
 }
 
 template <int ACC, typename TS>
-__global__ void __launch_bounds__(1024) k_grid_tiles(const uint32_t *__restrict__ words,
-                                                     const ulonglong2 *__restrict__ ranges,
-                                                     const TS *__restrict__ data, const float *__restrict__ dcf,
-                                                     int C, int stride, fr_tiling tl, float2 *__restrict__ out,
-                                                     size_t cells)
+__global__ void __launch_bounds__(64) k_grid_items(const fr_item *__restrict__ items, int64_t nitems,
+                                                   const fr_unit *__restrict__ units,
+                                                   const uint2 *__restrict__ H, const uint2 *__restrict__ T,
+                                                   const TS *__restrict__ data, const float *__restrict__ dcf,
+                                                   int C, int stride, int acc_floats, int N, size_t cells,
+                                                   float2 *__restrict__ out, float2 *__restrict__ partial,
+                                                   int64_t nslots, int cells_max)
 {
-    extern __shared__ float acc[];
-    const int ncell = tl.tx * tl.ty * tl.tz;
-    for (int i = threadIdx.x; i < ncell * stride; i += blockDim.x) acc[i] = 0.f;
-    __syncthreads();
+    // LDS: accumulator [cells_max][stride] | tap stage [STAGE*64] uint2 | meta [2][64] uint32
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= nitems) return;
+    float *acc = lds;
+    uint2 *stage = reinterpret_cast<uint2 *>(lds + acc_floats);
+    uint32_t *metaA = reinterpret_cast<uint32_t *>(stage + 64 * STAGE);
+    uint32_t *metaB = metaA + 64;
 
-    const int tile = blockIdx.x;
+    const fr_item it = items[i];
+    const fr_unit U = units[it.unit];
+    const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
+    const int ncell = ux * uy * uz;
+    for (int q = lane; q < ncell * ACC_STRIDE; q += 64) acc[q] = 0.f;
+
     const int c0 = blockIdx.y * 32;
     const int nc = min(32, C - c0);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const bool active = lane < 2 * nc;
-    const int coil = c0 + (lane >> 1);
-    const int comp = lane & 1;
+    // lanes past the chunk's 2*nc components read a valid component and
+    // accumulate into their own columns of the row, which are never read
+    const int lc = lane < 2 * nc ? 2 * c0 + lane : 2 * c0;
+    const uint32_t vz = opaque_zero();
+    char *accb = reinterpret_cast<char *>(acc);
+    const uint32_t lane4 = 4u * lane;
 
-    const ulonglong2 r = ranges[(size_t)tile * tl.slabs + wave];
-    uint64_t b = r.x;
-    const uint64_t e = r.y;
-    while (b < e) {
-        const uint32_t s = words[b];
-        const uint32_t n = words[b + 1];
-        float d = 0.f;
-        if (active) d = __fmul_rn(dcf[s], load_comp<TS>(data, s, C, coil, comp));
-        const uint32_t *t = words + b + 2;
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t cell = t[2 * k];
-            const float w = __uint_as_float(t[2 * k + 1]);
-            const float v = __fmul_rn(w, d);
-            if (active) {
-                float *a = acc + cell * stride + lane;
-                if constexpr (ACC == 0) {
-                    atomicAdd(a, v);           // ds_add_f32; the slab's wave is the only writer
-                } else {
-                    *a = __fadd_rn(*a, v);
-                }
-            }
+    const uint64_t re = it.r1;
+    uint64_t nb = it.r0, ntb = it.t0;
+    if (nb < re) {
+        // Software pipeline, unrolled twice so buffers A and B never copy:
+        // batch B's loads are in flight while batch A is applied.
+        uint2 h = H[nb + lane];
+        batch_regs<TS> A, B;
+        prep(A, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, metaA, lane);
+        while (true) {
+            prep(B, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, metaB, lane);
+            compute<ACC>(A, metaA, stage, accb, lane4);
+            if (B.k == 0) break;
+            prep(A, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, metaA, lane);
+            compute<ACC>(B, metaB, stage, accb, lane4);
+            if (A.k == 0) break;
         }
-        b += 2 + 2 * (uint64_t)n;
     }
-    __syncthreads();
 
-    // write the tile: coil-major grids [C][N^dim], x fastest
-    const int ti = tile % tl.ntx;
-    const int tj = (tile / tl.ntx) % tl.nty;
-    const int tk = tile / (tl.ntx * tl.nty);
-    const int N = tl.N;
-    for (int i = threadIdx.x; i < nc * ncell; i += blockDim.x) {
-        const int c = i / ncell;
-        const int cell = i - c * ncell;
-        const int lx = cell % tl.tx;
-        const int ly = (cell / tl.tx) % tl.ty;
-        const int lz = cell / (tl.tx * tl.ty);
-        const int gx = ti * tl.tx + lx, gy = tj * tl.ty + ly, gz = tk * tl.tz + lz;
-        if (gx < N && gy < N && gz < (tl.dim == 3 ? N : 1)) {
-            const size_t g = ((size_t)gz * N + gy) * N + gx;
-            out[(size_t)(c0 + c) * cells + g] = make_float2(acc[cell * stride + 2 * c], acc[cell * stride + 2 * c + 1]);
+    if (it.slot < 0) {
+        // coil-major grids [C][N^dim], x fastest
+        for (int q = lane; q < nc * ncell; q += 64) {
+            const int c = q / ncell;
+            const int cell = q - c * ncell;
+            const int lx = cell % ux;
+            const int ly = (cell / ux) % uy;
+            const int lz = cell / (ux * uy);
+            const size_t g = ((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx);
+            out[(size_t)(c0 + c) * cells + g] = make_float2(acc[cell * ACC_STRIDE + 2 * c], acc[cell * ACC_STRIDE + 2 * c + 1]);
         }
+    } else {
+        float2 *pp = partial + ((size_t)blockIdx.y * nslots + it.slot) * (size_t)(32 * cells_max);
+        for (int q = lane; q < nc * ncell; q += 64) {
+            const int c = q / ncell;
+            const int cell = q - c * ncell;
+            pp[(size_t)c * cells_max + cell] = make_float2(acc[cell * ACC_STRIDE + 2 * c], acc[cell * ACC_STRIDE + 2 * c + 1]);
+        }
+    }
+}
+
+// Heavy units: sum their slices' partial boxes in slice order, write the grid.
+__global__ void k_combine(const fr_heavy *__restrict__ heavy, const fr_unit *__restrict__ units,
+                          const float2 *__restrict__ partial, int64_t nslots, int cells_max, int C, int N, size_t cells,
+                          float2 *__restrict__ out)
+{
+    const fr_heavy h = heavy[blockIdx.x];
+    const fr_unit U = units[h.unit];
+    const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
+    const int ncell = ux * uy * uz;
+    const int c0 = blockIdx.y * 32;
+    const int nc = min(32, C - c0);
+    const size_t slot_stride = (size_t)32 * cells_max;
+    const float2 *base = partial + ((size_t)blockIdx.y * nslots + h.slot0) * slot_stride;
+    for (int q = threadIdx.x; q < nc * ncell; q += blockDim.x) {
+        const int c = q / ncell;
+        const int cell = q - c * ncell;
+        float2 a = make_float2(0.f, 0.f);
+        for (uint32_t k = 0; k < h.nitems; ++k) {
+            const float2 v = base[k * slot_stride + (size_t)c * cells_max + cell];
+            a.x = __fadd_rn(a.x, v.x);
+            a.y = __fadd_rn(a.y, v.y);
+        }
+        const int lx = cell % ux;
+        const int ly = (cell / ux) % uy;
+        const int lz = cell / (ux * uy);
+        const size_t g = ((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx);
+        out[(size_t)(c0 + c) * cells + g] = a;
     }
 }
 
@@ -112,15 +273,37 @@ template <int ACC, typename TS>
 fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
 {
     fr_ctx *ctx = p->ctx;
-    const fr_tiling &tl = p->tl;
-    const int stride = 2 * std::min(s->C, 32) + 1;
-    const size_t lds = (size_t)tl.tile_cells() * stride * sizeof(float);
-    auto kern = k_grid_tiles<ACC, TS>;
+    const fr_geom &g = p->geom;
+    const int stride = ACC_STRIDE;
+    const int acc_floats = g.cells_max * stride;
+    const size_t lds = (size_t)acc_floats * sizeof(float) + sizeof(uint2) * 64 * STAGE + sizeof(uint32_t) * 128;
+    const int chunks = (s->C + 31) / 32;
+    if (p->nslots > 0) {
+        const size_t need = (size_t)chunks * p->nslots * 32 * g.cells_max * sizeof(float2);
+        if (need > p->partial_bytes) {
+            FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            hipFree(p->partial);
+            p->partial = nullptr;
+            p->partial_bytes = 0;
+            fr_status e = fr_alloc(ctx, (void **)&p->partial, need);
+            if (e) return e;
+            p->partial_bytes = need;
+        }
+    }
+    auto kern = k_grid_items<ACC, TS>;
     FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    dim3 grid(tl.tiles(), (s->C + 31) / 32);
-    hipLaunchKernelGGL(kern, grid, dim3(64 * tl.slabs), lds, ctx->stream, p->words, p->ranges,
-                       (const TS *)s->data, s->dcf, s->C, stride, tl, out, fr_cells(p->dim, p->N));
+    if (p->nitems == 0) return FR_OK;
+    const size_t cells = fr_cells(p->dim, p->N);
+    dim3 grid((unsigned)p->nitems, chunks);
+    hipLaunchKernelGGL(kern, grid, dim3(64), lds, ctx->stream, p->items, p->nitems, p->units, p->hdrs, p->taps,
+                       (const TS *)s->data, s->dcf, s->C, stride, acc_floats, (int)p->N, cells, out,
+                       (float2 *)p->partial, p->nslots, g.cells_max);
     FR_HIP(ctx, hipGetLastError());
+    if (p->nheavy > 0) {
+        hipLaunchKernelGGL(k_combine, dim3((unsigned)p->nheavy, chunks), dim3(256), 0, ctx->stream, p->heavy,
+                           p->units, (const float2 *)p->partial, p->nslots, g.cells_max, s->C, (int)p->N, cells, out);
+        FR_HIP(ctx, hipGetLastError());
+    }
     return FR_OK;
 }
 
@@ -144,5 +327,6 @@ extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *
              "grids (dim %d, N %u, C %d) do not match plan/samples (dim %d, N %u, C %d)", out->dim, out->N, out->C,
              p->dim, p->N, s->C);
     FR_CHECK(ctx, s->has_dcf, FR_ERR_STATE, "DCF not set");
+    FR_CHECK(ctx, p->geom.cells_max <= 64, FR_ERR_INVALID, "plan unit box exceeds 64 cells");
     return fr_launch_grid(p, s, out->data, 0);
 }

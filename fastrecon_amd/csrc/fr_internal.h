@@ -42,16 +42,47 @@ struct fr_grids {
     float2 *data = nullptr;    // [C][cells]
 };
 
-// Geometry of the tile decomposition shared by plan and kernels.
-struct fr_tiling {
+// Accumulator row of one grid cell in LDS: 64 components (32 coils, re/im)
+// plus one pad word so rows start in different banks.
+#define FR_ACC_STRIDE 65
+
+// A work unit is a box of grid cells owned by one wave.  Its records (one
+// per sample that has taps in the box, ascending sample order) are H[hdr ..
+// hdr + nrec) and its taps T[tap .. tap + ntap).
+struct fr_unit {
+    int32_t ox, oy, oz;          // origin (global cell coordinates)
+    int32_t ext;                 // ux | uy << 8 | uz << 16
+    uint64_t hdr;                // first record
+    uint64_t tap;                // first tap
+    uint32_t nrec, ntap;
+};
+
+// Unit geometry shared by plan and kernels.
+struct fr_geom {
     int dim;
     int N;
-    int tx, ty, tz;            // tile extent (tz = 1 in 2D)
-    int ntx, nty, ntz;         // tiles per axis
-    int slabs;                 // waves per workgroup; slab = slow-axis band
-    int slab_depth;            // slow-axis cells per slab
-    __host__ __device__ int tiles() const { return ntx * nty * ntz; }
-    __host__ __device__ int tile_cells() const { return tx * ty * tz; }
+    int bx, by, bz;              // base unit box (before splitting)
+    int cells_max;               // bx * by * bz
+    int waves;                   // waves per workgroup
+};
+
+// A work item is one wave's share of a unit: records [r0, r1) starting at
+// tap t0.  Light units have one item that writes the grid; heavy units are
+// cut by sample range into several items that write partial boxes to
+// scratch slot `slot`, summed afterwards in item order.
+struct fr_item {
+    uint32_t unit;
+    int32_t slot;                // -1: write the grid directly
+    uint64_t r0, r1, t0;
+};
+
+// A heavy unit: its items are [item0, item0 + nitems), partial slots
+// [slot0, slot0 + nitems).
+struct fr_heavy {
+    uint32_t unit;
+    uint32_t nitems;
+    uint32_t slot0;
+    uint32_t pad;
 };
 
 struct fr_plan {
@@ -63,11 +94,21 @@ struct fr_plan {
     int L = 0;
     float *lut = nullptr;              // device copy of the LUT
     std::vector<float> host_lut;
-    fr_tiling tl{};
+    fr_geom geom{};
     int accumulate = 0;
-    uint32_t *words = nullptr;         // record streams
-    ulonglong2 *ranges = nullptr;      // [tiles*slabs] (begin, end) word offsets
-    int64_t nwords = 0, nrecords = 0, ntaps = 0;
+    fr_unit *units = nullptr;          // device, heaviest first
+    int64_t nunits = 0;
+    uint2 *hdrs = nullptr;             // records: (sample, ntaps)
+    uint2 *taps = nullptr;             // taps: (LDS byte offset of the cell's row, LUT weight bits)
+    int64_t nrecords = 0, ntaps = 0;
+    int64_t max_unit_taps = 0;
+    int exact_order = 0;               // 1: no sample slicing (bitwise CPU order)
+    fr_item *items = nullptr;          // device, launch order
+    int64_t nitems = 0;
+    fr_heavy *heavy = nullptr;         // device
+    int64_t nheavy = 0, nslots = 0;
+    float *partial = nullptr;          // scratch for heavy-unit slices
+    size_t partial_bytes = 0;
     double build_ms = 0.0;
 };
 

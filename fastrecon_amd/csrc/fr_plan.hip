@@ -1,17 +1,23 @@
-// Gridding plan: per-(tile, slab) record streams of taps.
+// Gridding plan: adaptive work units and their record streams.
 //
-// Replaces cuGridLut::plan (ReconData/cuGridLut.cu:151-226).  The reference
-// materialises a (cell, {sample, distance}) tuple for every box cell of
-// every sample, sorts by cell and streams 25 partitions over PCIe per coil.
-// Here the grid is cut into tiles; each tile is owned by one workgroup whose
-// waves each own a slab (a band of the slowest tile axis).  For every
-// (tile, slab) the plan holds one stream of records, in ascending sample
-// order:
-//     [sample index][n] then n x [cell index inside the tile][LUT weight]
-// The stream order is produced by a stable radix sort of (key, sample)
-// pairs generated in sample order, so every grid cell later receives its
-// contributions in the reference CPU order.  The plan lives in HBM and is
-// reused for every coil, frame and DCF.
+// Replaces cuGridLut::plan (ReconData/cuGridLut.cu:151-226), which builds a
+// (cell, {sample, distance}) tuple for every box cell of every sample, sorts
+// it by cell and streams 25 partitions of it over PCIe for every coil.
+//
+// Here the grid is covered by "units", boxes of cells that one wave owns
+// (its accumulator lives in that wave's slice of LDS).  Units start as base
+// boxes sized to the LDS budget and are split in halves (a k-d split driven
+// by a per-cell tap histogram) until no unit carries more than a threshold
+// number of taps, so the 1/|k|^2 density peak of a radial trajectory is
+// spread over many waves; units are ordered heaviest first.  For every unit
+// the plan holds its records, in ascending sample order:
+//     H[r] = (sample, n),  T[t .. t+n) = (LDS byte offset of the cell's
+//                                          accumulator row, LUT weight)
+// Records come from a stable radix sort of (unit, sample) pairs generated in
+// sample order, so each cell later receives its contributions in the
+// reference CPU order (GridLut.cpp:96-145).  Tap sets and LUT indices use the
+// reference arithmetic (fr_taps.h).  The plan depends only on the
+// trajectory, lives in HBM and serves every coil, frame and DCF.
 #include "fr_internal.h"
 #include "fr_taps.h"
 
@@ -19,132 +25,225 @@
 
 #include <algorithm>
 #include <chrono>
+#include <numeric>
 
 namespace {
 
-struct key_ctx {
-    fr_tiling tl;
+struct gctx {
     float hw;
     int L;
     int dim;
     int N;
 };
 
-// Enumerate the (tile, slab) keys a sample's box touches and the taps in
-// each: f(key, lo, hi) is called per key with the sub-box of the tile/slab.
-template <typename F>
-__device__ __forceinline__ void for_each_key(const key_ctx &k, const fr_box &b, F &&f)
+__device__ __forceinline__ size_t cell_index(int x, int y, int z, int N)
 {
-    const fr_tiling &tl = k.tl;
-    if (b.s[0] > b.e[0] || b.s[1] > b.e[1] || b.s[2] > b.e[2]) return;
-    const int slow = k.dim == 3 ? 2 : 1;
-    const int txs = b.s[0] / tl.tx, txe = b.e[0] / tl.tx;
-    const int tys = b.s[1] / tl.ty, tye = b.e[1] / tl.ty;
-    const int tzs = b.s[2] / tl.tz, tze = b.e[2] / tl.tz;
-    const int text[3] = {tl.tx, tl.ty, tl.tz};
-    for (int tz = tzs; tz <= tze; ++tz)
-        for (int ty = tys; ty <= tye; ++ty)
-            for (int tx = txs; tx <= txe; ++tx) {
-                const int t[3] = {tx, ty, tz};
-                int lo[3], hi[3];
-                for (int a = 0; a < 3; ++a) {
-                    lo[a] = max(b.s[a], t[a] * text[a]);
-                    hi[a] = min(b.e[a], t[a] * text[a] + text[a] - 1);
-                }
-                const int tile = (tz * tl.nty + ty) * tl.ntx + tx;
-                // split the slow axis into slabs
-                const int base = t[slow] * text[slow];
-                const int sl0 = (lo[slow] - base) / tl.slab_depth;
-                const int sl1 = (hi[slow] - base) / tl.slab_depth;
-                for (int sl = sl0; sl <= sl1; ++sl) {
-                    int l2[3] = {lo[0], lo[1], lo[2]}, h2[3] = {hi[0], hi[1], hi[2]};
-                    l2[slow] = max(lo[slow], base + sl * tl.slab_depth);
-                    h2[slow] = min(hi[slow], base + sl * tl.slab_depth + tl.slab_depth - 1);
-                    f(tile * tl.slabs + sl, t, l2, h2);
-                }
+    return ((size_t)z * N + y) * N + x;
+}
+
+// per-cell histogram of accepted taps
+__global__ void k_count_cells(const float *__restrict__ traj, int64_t M, gctx g, uint32_t *__restrict__ cnt)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    fr_box b;
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    fr_box_taps(b, b.s, b.e, g.hw, g.L, [&](int x, int y, int z, int) {
+        atomicAdd(&cnt[cell_index(x, y, z, g.N)], 1u);
+    });
+}
+
+// Visit the units a sample's box overlaps: a unit is met for the first time
+// at the first cell of (box ∩ unit) in z, y, x order, i.e. where each
+// coordinate equals max(box start, unit origin).  f(unit, lo, hi) gets the
+// sub-box box ∩ unit.
+template <typename F>
+__device__ __forceinline__ void for_each_unit(const fr_box &b, int N, const int32_t *__restrict__ cell_unit,
+                                              const fr_unit *__restrict__ units, F &&f)
+{
+    for (int z = b.s[2]; z <= b.e[2]; ++z)
+        for (int y = b.s[1]; y <= b.e[1]; ++y)
+            for (int x = b.s[0]; x <= b.e[0]; ++x) {
+                const int u = cell_unit[cell_index(x, y, z, N)];
+                const fr_unit U = units[u];
+                const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
+                if (x != max(b.s[0], U.ox) || y != max(b.s[1], U.oy) || z != max(b.s[2], U.oz)) continue;
+                const int lo[3] = {x, y, z};
+                const int hi[3] = {min(b.e[0], U.ox + ux - 1), min(b.e[1], U.oy + uy - 1), min(b.e[2], U.oz + uz - 1)};
+                f(u, U, lo, hi);
             }
 }
 
-__global__ void k_count_records(const float *__restrict__ traj, int64_t M, key_ctx k,
-                                uint32_t *__restrict__ nrec, uint32_t *__restrict__ ntap)
+__global__ void k_count_records(const float *__restrict__ traj, int64_t M, gctx g,
+                                const int32_t *__restrict__ cell_unit, const fr_unit *__restrict__ units,
+                                uint32_t *__restrict__ nrec)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     fr_box b;
-    fr_sample_box(k.dim, k.N, traj, M, s, k.hw, b);
-    uint32_t r = 0, t = 0;
-    for_each_key(k, b, [&](int key, const int *tcoord, const int *lo, const int *hi) {
-        uint32_t n = 0;
-        fr_box_taps(b, lo, hi, k.hw, k.L, [&](int, int, int, int) { ++n; });
-        if (n) { ++r; t += n; }
-    });
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    uint32_t r = 0;
+    if (b.s[0] <= b.e[0] && b.s[1] <= b.e[1] && b.s[2] <= b.e[2])
+        for_each_unit(b, g.N, cell_unit, units, [&](int, const fr_unit &, const int *lo, const int *hi) {
+            uint32_t n = 0;
+            fr_box_taps(b, lo, hi, g.hw, g.L, [&](int, int, int, int) { ++n; });
+            r += n ? 1u : 0u;
+        });
     nrec[s] = r;
-    ntap[s] = t;
 }
 
-__global__ void k_emit_records(const float *__restrict__ traj, int64_t M, key_ctx k,
+__global__ void k_emit_records(const float *__restrict__ traj, int64_t M, gctx g,
+                               const int32_t *__restrict__ cell_unit, const fr_unit *__restrict__ units,
                                const uint64_t *__restrict__ pos, uint32_t *__restrict__ keys,
                                uint64_t *__restrict__ vals)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     fr_box b;
-    fr_sample_box(k.dim, k.N, traj, M, s, k.hw, b);
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
     uint64_t p = pos[s];
-    for_each_key(k, b, [&](int key, const int *tcoord, const int *lo, const int *hi) {
-        uint32_t n = 0;
-        fr_box_taps(b, lo, hi, k.hw, k.L, [&](int, int, int, int) { ++n; });
-        if (n) {
-            keys[p] = (uint32_t)key;
-            vals[p] = ((uint64_t)s << 32) | n;
-            ++p;
-        }
-    });
+    if (b.s[0] <= b.e[0] && b.s[1] <= b.e[1] && b.s[2] <= b.e[2])
+        for_each_unit(b, g.N, cell_unit, units, [&](int u, const fr_unit &, const int *lo, const int *hi) {
+            uint32_t n = 0;
+            fr_box_taps(b, lo, hi, g.hw, g.L, [&](int, int, int, int) { ++n; });
+            if (n) {
+                keys[p] = (uint32_t)u;
+                vals[p] = ((uint64_t)s << 32) | n;
+                ++p;
+            }
+        });
 }
 
-__global__ void k_record_words(const uint64_t *__restrict__ vals, int64_t R, uint64_t *__restrict__ words)
+__global__ void k_record_taps(const uint64_t *__restrict__ vals, int64_t R, uint64_t *__restrict__ n)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R) words[r] = 2 + 2 * (vals[r] & 0xffffffffu);
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) n[r] = vals[r] & 0xffffffffu;
 }
 
-__global__ void k_key_ranges(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ off,
-                             const uint64_t *__restrict__ vals, int64_t R, ulonglong2 *__restrict__ ranges)
+__global__ void k_unit_ranges(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ toff, int64_t R,
+                              fr_unit *__restrict__ units)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    uint32_t k = keys[r];
-    if (r == 0 || keys[r - 1] != k) ranges[k].x = off[r];
-    if (r == R - 1 || keys[r + 1] != k) ranges[k].y = off[r] + 2 + 2 * (vals[r] & 0xffffffffu);
+    const uint32_t k = keys[r];
+    if (r == 0 || keys[r - 1] != k) {
+        units[k].hdr = (uint64_t)r;
+        units[k].tap = toff[r];
+    }
+    if (r == R - 1 || keys[r + 1] != k) {
+        // the first record of this unit was written by another thread; the
+        // counts are finished by k_unit_counts once all ranges are known
+        units[k].nrec = (uint32_t)(r + 1);          // provisional: one past the last record
+        units[k].ntap = (uint32_t)0;
+    }
 }
 
-__global__ void k_fill_records(const float *__restrict__ traj, int64_t M, key_ctx k,
-                               const uint32_t *__restrict__ keys, const uint64_t *__restrict__ vals,
-                               const uint64_t *__restrict__ off, int64_t R, const float *__restrict__ lut,
-                               uint32_t *__restrict__ words)
+__global__ void k_unit_counts(fr_unit *__restrict__ units, int64_t nunits, const uint64_t *__restrict__ toff)
 {
-    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    fr_unit U = units[u];
+    if (U.nrec == 0) return;                         // no records
+    const uint64_t end = U.nrec;                     // one past the last record
+    U.nrec = (uint32_t)(end - U.hdr);
+    U.ntap = (uint32_t)(toff[end] - U.tap);
+    units[u] = U;
+}
+
+__global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g,
+                               const int32_t *__restrict__ cell_unit, const fr_unit *__restrict__ units,
+                               const uint32_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+                               const uint64_t *__restrict__ toff, int64_t R, const float *__restrict__ lut,
+                               uint2 *__restrict__ H, uint2 *__restrict__ T)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const uint32_t key = keys[r];
     const int64_t s = (int64_t)(vals[r] >> 32);
     const uint32_t n = (uint32_t)(vals[r] & 0xffffffffu);
+    H[r] = make_uint2((uint32_t)s, n);
     fr_box b;
-    fr_sample_box(k.dim, k.N, traj, M, s, k.hw, b);
-    uint32_t *w = words + off[r];
-    w[0] = (uint32_t)s;
-    w[1] = n;
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    uint2 *w = T + toff[r];
     uint32_t i = 0;
-    const fr_tiling tl = k.tl;
-    for_each_key(k, b, [&](int kk, const int *t, const int *lo, const int *hi) {
-        if ((uint32_t)kk != key) return;
-        const int ox = t[0] * tl.tx, oy = t[1] * tl.ty, oz = t[2] * tl.tz;
-        fr_box_taps(b, lo, hi, k.hw, k.L, [&](int x, int y, int z, int ki) {
-            uint32_t cell = (uint32_t)(((z - oz) * tl.ty + (y - oy)) * tl.tx + (x - ox));
-            w[2 + 2 * i] = cell;
-            w[3 + 2 * i] = __float_as_uint(lut[ki]);
-            ++i;
+    for_each_unit(b, g.N, cell_unit, units, [&](int u, const fr_unit &U, const int *lo, const int *hi) {
+        if ((uint32_t)u != key) return;
+        const int ux = U.ext & 255, uy = (U.ext >> 8) & 255;
+        fr_box_taps(b, lo, hi, g.hw, g.L, [&](int x, int y, int z, int ki) {
+            const uint32_t cell = (uint32_t)(((z - U.oz) * uy + (y - U.oy)) * ux + (x - U.ox));
+            w[i++] = make_uint2(cell * (4u * FR_ACC_STRIDE), __float_as_uint(lut[ki]));
         });
     });
+}
+
+// A record opens a new work item when it is its unit's first record or when
+// the unit's running tap count crosses a multiple of the slice size.
+__global__ void k_item_flags(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ toff, int64_t R,
+                             const fr_unit *__restrict__ units, uint64_t slice, uint32_t *__restrict__ flag)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const fr_unit U = units[keys[r]];
+    uint32_t f = (uint64_t)r == U.hdr;
+    if (!f) f = (toff[r] - U.tap) / slice != (toff[r - 1] - U.tap) / slice;
+    flag[r] = f;
+}
+
+__global__ void k_make_items(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ toff,
+                             const uint32_t *__restrict__ flag, const uint32_t *__restrict__ idx, int64_t R,
+                             const fr_unit *__restrict__ units, fr_item *__restrict__ items,
+                             uint32_t *__restrict__ first_item, uint32_t *__restrict__ last_item)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R || !flag[r]) return;
+    const uint32_t u = keys[r];
+    const fr_unit U = units[u];
+    const uint32_t i = idx[r] - 1;                    // inclusive scan of flags
+    // the item ends at the next flagged record of the same unit or the unit end
+    uint64_t r1 = U.hdr + U.nrec;
+    for (uint64_t q = (uint64_t)r + 1; q < U.hdr + U.nrec; ++q)
+        if (flag[q]) { r1 = q; break; }
+    items[i] = fr_item{u, -1, (uint64_t)r, r1, toff[r]};
+    if ((uint64_t)r == U.hdr) first_item[u] = i;
+    if (r1 == U.hdr + U.nrec) last_item[u] = i;
+}
+
+// empty units still own cells: give each one item that writes zeros
+__global__ void k_empty_items(const fr_unit *__restrict__ units, int64_t nunits, const uint32_t *__restrict__ eidx,
+                              uint32_t base, fr_item *__restrict__ items)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits || units[u].nrec != 0) return;
+    items[base + eidx[u] - 1] = fr_item{(uint32_t)u, -1, 0, 0, 0};
+}
+
+__global__ void k_empty_flags(const fr_unit *__restrict__ units, int64_t nunits, uint32_t *__restrict__ f)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nunits) f[u] = units[u].nrec == 0;
+}
+
+__global__ void k_heavy_flags(const fr_unit *__restrict__ units, int64_t nunits, const uint32_t *__restrict__ first,
+                              const uint32_t *__restrict__ last, uint32_t *__restrict__ f, uint32_t *__restrict__ nsl)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const bool h = units[u].nrec != 0 && last[u] != first[u];
+    f[u] = h;
+    nsl[u] = h ? last[u] - first[u] + 1 : 0;
+}
+
+__global__ void k_make_heavy(int64_t nunits, const uint32_t *__restrict__ hflag, const uint32_t *__restrict__ hidx,
+                             const uint32_t *__restrict__ nsl, const uint32_t *__restrict__ slot_scan,
+                             const uint32_t *__restrict__ first, fr_heavy *__restrict__ heavy,
+                             fr_item *__restrict__ items)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits || !hflag[u]) return;
+    const uint32_t h = hidx[u] - 1;
+    const uint32_t slot0 = slot_scan[u] - nsl[u];       // exclusive
+    heavy[h] = fr_heavy{(uint32_t)u, nsl[u], slot0, 0};
+    for (uint32_t k = 0; k < nsl[u]; ++k) items[first[u] + k].slot = (int32_t)(slot0 + k);
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -155,75 +254,169 @@ struct dbuf {
     ~dbuf() { if (p) (void)hipFree(p); }
 };
 
+int floor_pow2(int v)
+{
+    int p = 1;
+    while (p * 2 <= v) p *= 2;
+    return p;
+}
+
 }  // namespace
 
-// Choose the tile decomposition.  The LDS accumulator holds, per tile cell,
-// 2 * min(C, 32) floats plus one pad float (conflict-free write-out).
-static fr_tiling choose_tiling(int dim, unsigned N, int coils, const fr_tune *tune)
+// Base unit box: 64 cells (8 x 8 in a plane), x fastest for coalesced
+// write-out.  A unit never exceeds 64 cells, so a record (one sample's taps
+// inside one unit) never carries more than 64 taps.
+static fr_geom choose_geom(int dim, unsigned N, int coils, const fr_tune *tune)
 {
-    fr_tiling t{};
-    t.dim = dim;
-    t.N = (int)N;
-    const int P = 2 * std::min(coils, 32);
-    if (dim == 3) {
-        t.tx = t.ty = t.tz = P <= 4 ? 16 : 8;
-        t.slabs = t.tz;
-    } else {
-        t.tx = P <= 4 ? 64 : 32;
-        t.ty = P <= 4 ? 64 : 32;
-        t.tz = 1;
-        t.slabs = 16;
-    }
+    (void)coils;
+    fr_geom g{};
+    g.dim = dim;
+    g.N = (int)N;
+    g.bx = 8;
+    g.by = 8;
+    g.bz = 1;
+    g.waves = 1;
     if (tune) {
-        if (tune->tile_x > 0) t.tx = tune->tile_x;
-        if (tune->tile_y > 0) t.ty = tune->tile_y;
-        if (dim == 3 && tune->tile_z > 0) t.tz = tune->tile_z;
-        if (tune->slabs > 0) t.slabs = tune->slabs;
+        if (tune->tile_x > 0) g.bx = tune->tile_x;
+        if (tune->tile_y > 0) g.by = tune->tile_y;
+        if (dim == 3 && tune->tile_z > 0) g.bz = tune->tile_z;
     }
-    const int slow = dim == 3 ? t.tz : t.ty;
-    if (t.slabs > slow) t.slabs = slow;
-    t.slab_depth = slow / t.slabs;
-    t.ntx = ((int)N + t.tx - 1) / t.tx;
-    t.nty = ((int)N + t.ty - 1) / t.ty;
-    t.ntz = dim == 3 ? ((int)N + t.tz - 1) / t.tz : 1;
-    return t;
+    g.bx = std::min(g.bx, (int)N);
+    g.by = std::min(g.by, (int)N);
+    g.bz = dim == 3 ? std::min(g.bz, (int)N) : 1;
+    g.cells_max = g.bx * g.by * g.bz;
+    return g;
 }
+
+namespace {
+
+struct host_unit {
+    int ox, oy, oz, ux, uy, uz;
+    uint64_t taps;
+};
+
+struct unit_builder {
+    const std::vector<uint32_t> &cnt;
+    int N, dim;
+    uint64_t thr;
+    std::vector<host_unit> out;
+
+    uint64_t sum(int ox, int oy, int oz, int ux, int uy, int uz) const
+    {
+        uint64_t t = 0;
+        for (int z = oz; z < oz + uz; ++z)
+            for (int y = oy; y < oy + uy; ++y) {
+                const uint32_t *row = cnt.data() + ((size_t)z * N + y) * N + ox;
+                for (int x = 0; x < ux; ++x) t += row[x];
+            }
+        return t;
+    }
+
+    void split(host_unit b)
+    {
+        if (b.taps <= thr || b.ux * b.uy * b.uz == 1) {
+            out.push_back(b);
+            return;
+        }
+        host_unit a = b, c = b;
+        if (b.uz >= b.uy && b.uz >= b.ux && b.uz > 1) {
+            a.uz = b.uz / 2; c.oz = b.oz + a.uz; c.uz = b.uz - a.uz;
+        } else if (b.uy >= b.ux && b.uy > 1) {
+            a.uy = b.uy / 2; c.oy = b.oy + a.uy; c.uy = b.uy - a.uy;
+        } else {
+            a.ux = b.ux / 2; c.ox = b.ox + a.ux; c.ux = b.ux - a.ux;
+        }
+        a.taps = sum(a.ox, a.oy, a.oz, a.ux, a.uy, a.uz);
+        c.taps = b.taps - a.taps;
+        split(a);
+        split(c);
+    }
+};
+
+}  // namespace
 
 fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
 {
     fr_ctx *ctx = p->ctx;
     hipStream_t st = ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
-    fr_tiling tl = choose_tiling(p->dim, p->N, s->C, tune);
-    const int slow = p->dim == 3 ? tl.tz : tl.ty;
-    FR_CHECK(ctx, tl.tx >= 1 && tl.ty >= 1 && tl.tz >= 1 && tl.slabs >= 1 && slow % tl.slabs == 0,
-             FR_ERR_INVALID, "tile %dx%dx%d with %d slabs is not valid", tl.tx, tl.ty, tl.tz, tl.slabs);
-    FR_CHECK(ctx, tl.slabs * 64 <= 1024, FR_ERR_INVALID, "at most 16 slabs per tile");
-    const int P = 2 * std::min(s->C, 32);
-    const int64_t lds = (int64_t)tl.tile_cells() * (P + 1) * 4;
-    FR_CHECK(ctx, lds <= 160 * 1024, FR_ERR_INVALID,
-             "tile %dx%dx%d needs %lld B of LDS for %d coils (max 163840)", tl.tx, tl.ty, tl.tz,
-             (long long)lds, std::min(s->C, 32));
-    const int64_t nkeys = (int64_t)tl.tiles() * tl.slabs;
-    FR_CHECK(ctx, nkeys < (int64_t(1) << 31), FR_ERR_UNSUPPORTED, "too many tiles");
-    p->tl = tl;
+    fr_geom geom = choose_geom(p->dim, p->N, s->C, tune);
+    const int stride = 2 * std::min(s->C, 32) + 1;
+    FR_CHECK(ctx, geom.bx >= 1 && geom.by >= 1 && geom.bz >= 1 && geom.bx <= 255 && geom.by <= 255 && geom.bz <= 255,
+             FR_ERR_INVALID, "unit box %dx%dx%d out of range", geom.bx, geom.by, geom.bz);
+    FR_CHECK(ctx, geom.cells_max <= 64, FR_ERR_INVALID, "unit box %dx%dx%d exceeds 64 cells", geom.bx, geom.by,
+             geom.bz);
+    (void)stride;
+    p->geom = geom;
     p->accumulate = tune ? tune->accumulate : 0;
+    p->exact_order = tune ? (tune->exact_order != 0) : 0;
 
-    key_ctx k{tl, p->kHW, p->L, p->dim, (int)p->N};
+    const int N = (int)p->N;
+    const size_t cells = fr_cells(p->dim, p->N);
     const int64_t M = s->M;
-    dbuf<uint32_t> nrec, ntap;
-    dbuf<uint64_t> pos;
+    gctx g{p->kHW, p->L, p->dim, N};
     fr_status e;
-    if ((e = fr_alloc(ctx, (void **)&nrec.p, 4 * M)) || (e = fr_alloc(ctx, (void **)&ntap.p, 4 * M)) ||
-        (e = fr_alloc(ctx, (void **)&pos.p, 8 * (M + 1))))
-        return e;
-    hipLaunchKernelGGL(k_count_records, dim3(nblk(M, 256)), dim3(256), 0, st, s->traj, M, k, nrec.p, ntap.p);
-    FR_HIP(ctx, hipGetLastError());
 
-    // exclusive scan of record counts -> positions (64-bit)
-    size_t tmp_bytes = 0;
+    // 1. per-cell tap histogram
+    dbuf<uint32_t> cnt_d;
+    if ((e = fr_alloc(ctx, (void **)&cnt_d.p, 4 * cells))) return e;
+    FR_HIP(ctx, hipMemsetAsync(cnt_d.p, 0, 4 * cells, st));
+    hipLaunchKernelGGL(k_count_cells, dim3(nblk(M, 256)), dim3(256), 0, st, s->traj, M, g, cnt_d.p);
+    FR_HIP(ctx, hipGetLastError());
+    std::vector<uint32_t> cnt(cells);
+    FR_HIP(ctx, hipMemcpyAsync(cnt.data(), cnt_d.p, 4 * cells, hipMemcpyDeviceToHost, st));
+    FR_HIP(ctx, hipStreamSynchronize(st));
+
+    // 2. units: base boxes split until each carries at most `thr` taps
+    uint64_t total = 0;
+    for (uint32_t c : cnt) total += c;
+    const int cus = std::max(1, ctx->cu_count);
+    uint64_t thr = std::max<uint64_t>(256, total / ((uint64_t)cus * 32));
+    if (tune && tune->split_taps > 0) thr = (uint64_t)tune->split_taps;
+    // exact order: heavy units are split in space (each cell keeps one wave
+    // and the CPU summation order); otherwise they are split by sample range
+    const uint64_t space_thr = p->exact_order ? thr : ~uint64_t(0);
+    const uint64_t slice_thr = p->exact_order ? ~uint64_t(0) : thr;
+    unit_builder ub{cnt, N, p->dim, space_thr, {}};
+    const int nz = p->dim == 3 ? N : 1;
+    for (int oz = 0; oz < nz; oz += geom.bz)
+        for (int oy = 0; oy < N; oy += geom.by)
+            for (int ox = 0; ox < N; ox += geom.bx) {
+                host_unit b{ox, oy, oz, std::min(geom.bx, N - ox), std::min(geom.by, N - oy),
+                            std::min(geom.bz, nz - oz), 0};
+                b.taps = ub.sum(b.ox, b.oy, b.oz, b.ux, b.uy, b.uz);
+                ub.split(b);
+            }
+    std::vector<host_unit> &hu = ub.out;
+    std::stable_sort(hu.begin(), hu.end(), [](const host_unit &a, const host_unit &b) { return a.taps > b.taps; });
+    const int64_t nunits = (int64_t)hu.size();
+    std::vector<fr_unit> units(nunits);
+    std::vector<int32_t> cell_unit(cells);
+    for (int64_t u = 0; u < nunits; ++u) {
+        const host_unit &h = hu[u];
+        units[u] = fr_unit{h.ox, h.oy, h.oz, h.ux | (h.uy << 8) | (h.uz << 16), 0, 0, 0, 0};
+        for (int z = h.oz; z < h.oz + h.uz; ++z)
+            for (int y = h.oy; y < h.oy + h.uy; ++y)
+                for (int x = h.ox; x < h.ox + h.ux; ++x) cell_unit[((size_t)z * N + y) * N + x] = (int32_t)u;
+    }
+    p->max_unit_taps = nunits ? (int64_t)hu[0].taps : 0;
+    dbuf<int32_t> cu_d;
+    if ((e = fr_alloc(ctx, (void **)&cu_d.p, 4 * cells))) return e;
+    if ((e = fr_alloc(ctx, (void **)&p->units, sizeof(fr_unit) * std::max<int64_t>(1, nunits)))) return e;
+    FR_HIP(ctx, hipMemcpyAsync(cu_d.p, cell_unit.data(), 4 * cells, hipMemcpyHostToDevice, st));
+    FR_HIP(ctx, hipMemcpyAsync(p->units, units.data(), sizeof(fr_unit) * nunits, hipMemcpyHostToDevice, st));
+    p->nunits = nunits;
+
+    // 3. records: (unit, sample) pairs in sample order, stable-sorted by unit
+    dbuf<uint32_t> nrec;
+    dbuf<uint64_t> pos;
+    if ((e = fr_alloc(ctx, (void **)&nrec.p, 4 * M)) || (e = fr_alloc(ctx, (void **)&pos.p, 8 * (M + 1)))) return e;
+    hipLaunchKernelGGL(k_count_records, dim3(nblk(M, 256)), dim3(256), 0, st, s->traj, M, g, cu_d.p, p->units,
+                       nrec.p);
+    FR_HIP(ctx, hipGetLastError());
     auto rec64 = hipcub::TransformInputIterator<uint64_t, hipcub::CastOp<uint64_t>, const uint32_t *>(
         nrec.p, hipcub::CastOp<uint64_t>());
+    size_t tmp_bytes = 0;
     FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, rec64, pos.p + 1, (int)M, st));
     dbuf<char> tmp;
     if ((e = fr_alloc(ctx, (void **)&tmp.p, tmp_bytes + 16))) return e;
@@ -240,12 +433,11 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     if ((e = fr_alloc(ctx, (void **)&keys_in.p, 4 * R + 4)) || (e = fr_alloc(ctx, (void **)&keys_out.p, 4 * R + 4)) ||
         (e = fr_alloc(ctx, (void **)&vals_in.p, 8 * R + 8)) || (e = fr_alloc(ctx, (void **)&vals_out.p, 8 * R + 8)))
         return e;
-    hipLaunchKernelGGL(k_emit_records, dim3(nblk(M, 256)), dim3(256), 0, st, s->traj, M, k, pos.p, keys_in.p,
-                       vals_in.p);
+    hipLaunchKernelGGL(k_emit_records, dim3(nblk(M, 256)), dim3(256), 0, st, s->traj, M, g, cu_d.p, p->units,
+                       pos.p, keys_in.p, vals_in.p);
     FR_HIP(ctx, hipGetLastError());
-
     int end_bit = 1;
-    while (end_bit < 32 && (int64_t(1) << end_bit) < nkeys) ++end_bit;
+    while (end_bit < 32 && (int64_t(1) << end_bit) < nunits) ++end_bit;
     size_t sort_bytes = 0;
     FR_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_in.p, keys_out.p, vals_in.p,
                                                    vals_out.p, (int)R, 0, end_bit, st));
@@ -254,35 +446,88 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     FR_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(sort_tmp.p, sort_bytes, keys_in.p, keys_out.p, vals_in.p,
                                                    vals_out.p, (int)R, 0, end_bit, st));
 
-    // word offsets of each record (exclusive scan of record sizes)
-    dbuf<uint64_t> rw, off;
-    if ((e = fr_alloc(ctx, (void **)&rw.p, 8 * R + 8)) || (e = fr_alloc(ctx, (void **)&off.p, 8 * R + 16)))
+    // 4. tap offsets (exclusive scan of per-record tap counts)
+    dbuf<uint64_t> rn, toff;
+    if ((e = fr_alloc(ctx, (void **)&rn.p, 8 * R + 8)) || (e = fr_alloc(ctx, (void **)&toff.p, 8 * R + 16)))
         return e;
-    hipLaunchKernelGGL(k_record_words, dim3(nblk(R, 256)), dim3(256), 0, st, vals_out.p, (int64_t)R, rw.p);
+    if (R) hipLaunchKernelGGL(k_record_taps, dim3(nblk(R, 256)), dim3(256), 0, st, vals_out.p, (int64_t)R, rn.p);
     size_t scan_bytes = 0;
-    FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, rw.p, off.p + 1, (int)R, st));
+    FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, rn.p, toff.p + 1, (int)R, st));
     dbuf<char> scan_tmp;
     if ((e = fr_alloc(ctx, (void **)&scan_tmp.p, scan_bytes + 16))) return e;
-    FR_HIP(ctx, hipMemsetAsync(off.p, 0, 8, st));
-    FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(scan_tmp.p, scan_bytes, rw.p, off.p + 1, (int)R, st));
-    uint64_t W = 0;
-    FR_HIP(ctx, hipMemcpyAsync(&W, off.p + R, 8, hipMemcpyDeviceToHost, st));
+    FR_HIP(ctx, hipMemsetAsync(toff.p, 0, 8, st));
+    if (R) FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(scan_tmp.p, scan_bytes, rn.p, toff.p + 1, (int)R, st));
+    uint64_t TT = 0;
+    FR_HIP(ctx, hipMemcpyAsync(&TT, toff.p + R, 8, hipMemcpyDeviceToHost, st));
     FR_HIP(ctx, hipStreamSynchronize(st));
 
-    if ((e = fr_alloc(ctx, (void **)&p->words, 4 * W + 16))) return e;
-    if ((e = fr_alloc(ctx, (void **)&p->ranges, sizeof(ulonglong2) * nkeys))) return e;
-    FR_HIP(ctx, hipMemsetAsync(p->ranges, 0, sizeof(ulonglong2) * nkeys, st));
-    if (R > 0) {
-        hipLaunchKernelGGL(k_key_ranges, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, off.p, vals_out.p,
-                           (int64_t)R, (ulonglong2 *)p->ranges);
-        hipLaunchKernelGGL(k_fill_records, dim3(nblk(R, 128)), dim3(128), 0, st, s->traj, M, k, keys_out.p,
-                           vals_out.p, off.p, (int64_t)R, p->lut, p->words);
+    // 5. unit record/tap ranges, then the records themselves
+    if ((e = fr_alloc(ctx, (void **)&p->hdrs, sizeof(uint2) * (R + 64)))) return e;
+    // padded so a wave may stage 512 taps from any 64-aligned chunk base
+    if ((e = fr_alloc(ctx, (void **)&p->taps, sizeof(uint2) * (TT + 576)))) return e;
+    FR_HIP(ctx, hipMemsetAsync(p->taps, 0, sizeof(uint2) * (TT + 576), st));
+    FR_HIP(ctx, hipMemsetAsync(p->hdrs, 0, sizeof(uint2) * (R + 64), st));
+    if (R) {
+        hipLaunchKernelGGL(k_unit_ranges, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, (int64_t)R,
+                           p->units);
+        hipLaunchKernelGGL(k_unit_counts, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, toff.p);
+        hipLaunchKernelGGL(k_fill_records, dim3(nblk(R, 128)), dim3(128), 0, st, s->traj, M, g, cu_d.p, p->units,
+                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, p->hdrs, p->taps);
     }
     FR_HIP(ctx, hipGetLastError());
-    FR_HIP(ctx, hipStreamSynchronize(st));
-    p->nwords = (int64_t)W;
+
+    // 6. work items: one per unit, heavy units cut into slices of slice_thr taps
+    {
+        dbuf<uint32_t> flag, idx, first, last, eflag, eidx, hflag, hidx, nsl, sscan;
+        const int64_t nu = std::max<int64_t>(1, nunits);
+        if ((e = fr_alloc(ctx, (void **)&flag.p, 4 * R + 4)) || (e = fr_alloc(ctx, (void **)&idx.p, 4 * R + 4)) ||
+            (e = fr_alloc(ctx, (void **)&first.p, 4 * nu)) || (e = fr_alloc(ctx, (void **)&last.p, 4 * nu)) ||
+            (e = fr_alloc(ctx, (void **)&eflag.p, 4 * nu)) || (e = fr_alloc(ctx, (void **)&eidx.p, 4 * nu)) ||
+            (e = fr_alloc(ctx, (void **)&hflag.p, 4 * nu)) || (e = fr_alloc(ctx, (void **)&hidx.p, 4 * nu)) ||
+            (e = fr_alloc(ctx, (void **)&nsl.p, 4 * nu)) || (e = fr_alloc(ctx, (void **)&sscan.p, 4 * nu)))
+            return e;
+        size_t b1 = 0, b2 = 0;
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, b1, flag.p, idx.p, (int)std::max<uint64_t>(R, 1), st));
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, b2, eflag.p, eidx.p, (int)nu, st));
+        dbuf<char> stmp;
+        if ((e = fr_alloc(ctx, (void **)&stmp.p, std::max(b1, b2) + 16))) return e;
+        uint32_t nrec_items = 0, nempty = 0, nheavy = 0, nslots = 0;
+        if (R) {
+            hipLaunchKernelGGL(k_item_flags, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, (int64_t)R,
+                               p->units, slice_thr, flag.p);
+            FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(stmp.p, b1, flag.p, idx.p, (int)R, st));
+            FR_HIP(ctx, hipMemcpyAsync(&nrec_items, idx.p + R - 1, 4, hipMemcpyDeviceToHost, st));
+        }
+        hipLaunchKernelGGL(k_empty_flags, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, eflag.p);
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(stmp.p, b2, eflag.p, eidx.p, (int)nunits, st));
+        FR_HIP(ctx, hipMemcpyAsync(&nempty, eidx.p + nunits - 1, 4, hipMemcpyDeviceToHost, st));
+        FR_HIP(ctx, hipStreamSynchronize(st));
+        p->nitems = (int64_t)nrec_items + nempty;
+        if ((e = fr_alloc(ctx, (void **)&p->items, sizeof(fr_item) * std::max<int64_t>(1, p->nitems)))) return e;
+        FR_HIP(ctx, hipMemsetAsync(first.p, 0, 4 * nu, st));
+        FR_HIP(ctx, hipMemsetAsync(last.p, 0, 4 * nu, st));
+        if (R)
+            hipLaunchKernelGGL(k_make_items, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, flag.p, idx.p,
+                               (int64_t)R, p->units, p->items, first.p, last.p);
+        hipLaunchKernelGGL(k_empty_items, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, eidx.p,
+                           nrec_items, p->items);
+        hipLaunchKernelGGL(k_heavy_flags, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, first.p,
+                           last.p, hflag.p, nsl.p);
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(stmp.p, b2, hflag.p, hidx.p, (int)nunits, st));
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(stmp.p, b2, nsl.p, sscan.p, (int)nunits, st));
+        FR_HIP(ctx, hipMemcpyAsync(&nheavy, hidx.p + nunits - 1, 4, hipMemcpyDeviceToHost, st));
+        FR_HIP(ctx, hipMemcpyAsync(&nslots, sscan.p + nunits - 1, 4, hipMemcpyDeviceToHost, st));
+        FR_HIP(ctx, hipStreamSynchronize(st));
+        p->nheavy = nheavy;
+        p->nslots = nslots;
+        if ((e = fr_alloc(ctx, (void **)&p->heavy, sizeof(fr_heavy) * std::max<uint32_t>(1, nheavy)))) return e;
+        hipLaunchKernelGGL(k_make_heavy, dim3(nblk(nunits, 256)), dim3(256), 0, st, nunits, hflag.p, hidx.p, nsl.p,
+                           sscan.p, first.p, p->heavy, p->items);
+        FR_HIP(ctx, hipGetLastError());
+        FR_HIP(ctx, hipStreamSynchronize(st));
+    }
     p->nrecords = (int64_t)R;
-    p->ntaps = ((int64_t)W - 2 * (int64_t)R) / 2;
+    p->ntaps = (int64_t)TT;
     p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FR_OK;
 }
@@ -318,7 +563,8 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
 {
     if (!p) return FR_OK;
     hipStreamSynchronize(p->ctx->stream);
-    hipFree(p->lut); hipFree(p->words); hipFree(p->ranges);
+    hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps);
+    hipFree(p->items); hipFree(p->heavy); hipFree(p->partial);
     delete p;
     return FR_OK;
 }
@@ -329,12 +575,17 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->samples = p->M;
     info->taps = p->ntaps;
     info->records = p->nrecords;
-    info->streams = (int64_t)p->tl.tiles() * p->tl.slabs;
-    info->tile_x = p->tl.tx; info->tile_y = p->tl.ty; info->tile_z = p->tl.tz;
-    info->slabs = p->tl.slabs;
-    info->tiles = p->tl.tiles();
+    info->streams = p->nunits;
+    info->tile_x = p->geom.bx; info->tile_y = p->geom.by; info->tile_z = p->geom.bz;
+    info->slabs = p->geom.waves;
+    info->tiles = (int)std::min<int64_t>(p->nunits, INT32_MAX);
     info->lds_bytes = 0;
-    info->plan_bytes = 4 * p->nwords + (int64_t)sizeof(ulonglong2) * info->streams;
+    info->plan_bytes = (int64_t)sizeof(uint2) * (p->nrecords + p->ntaps) + (int64_t)sizeof(fr_unit) * p->nunits;
     info->build_ms = p->build_ms;
+    info->items = p->nitems;
+    info->heavy_units = p->nheavy;
+    info->max_unit_taps = p->max_unit_taps;
+    info->exact_order = p->exact_order;
+    info->pad = 0;
     return FR_OK;
 }

@@ -6,6 +6,7 @@ library is missing, loading it raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -38,13 +39,16 @@ class HipError(RuntimeError):
 
 class fr_tune(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
-                ("accumulate", C.c_int)]
+                ("accumulate", C.c_int), ("split_taps", C.c_int),
+                ("exact_order", C.c_int)]
 
 
 class fr_plan_info(C.Structure):
     _fields_ = [("samples", C.c_int64), ("taps", C.c_int64), ("records", C.c_int64), ("streams", C.c_int64),
                 ("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
-                ("tiles", C.c_int), ("lds_bytes", C.c_int), ("plan_bytes", C.c_int64), ("build_ms", C.c_double)]
+                ("tiles", C.c_int), ("lds_bytes", C.c_int), ("plan_bytes", C.c_int64), ("build_ms", C.c_double),
+                ("items", C.c_int64), ("heavy_units", C.c_int64), ("max_unit_taps", C.c_int64),
+                ("exact_order", C.c_int), ("pad", C.c_int)]
 
 
 _lib = None
@@ -141,11 +145,16 @@ def normalize_traj(traj, N):
 class Context:
     def __init__(self, device=0):
         self.L = load()
+        self._children = weakref.WeakSet()
         h = _vp()
         st = self.L.fr_ctx_create(device, C.byref(h))
         if st:
             raise HipError(f"fr_ctx_create({device}) failed with status {st} (is a GPU visible?)")
         self.h = h
+
+    def adopt(self, child):
+        """Handles are destroyed before their context (they use its stream)."""
+        self._children.add(child)
 
     def check(self, st, what):
         if st:
@@ -160,6 +169,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             self.L.fr_ctx_destroy(self.h)
             self.h = None
 
@@ -176,6 +187,7 @@ class Samples:
         h = _vp()
         ctx.check(ctx.L.fr_samples_create(ctx.h, dim, self.M, self.C, sample_type, C.byref(h)), "fr_samples_create")
         self.h = h
+        ctx.adopt(self)
 
     def set_traj(self, traj):
         t = np.ascontiguousarray(traj, np.float32)
@@ -237,6 +249,7 @@ class Plan:
         ctx.check(ctx.L.fr_plan_create(ctx.h, samples.h, self.N, float(kwidth), _ptr(lut), len(lut),
                                        C.byref(t) if t is not None else None, C.byref(h)), "fr_plan_create")
         self.h = h
+        ctx.adopt(self)
 
     def info(self):
         i = fr_plan_info()
@@ -262,6 +275,7 @@ class Grids:
         h = _vp()
         ctx.check(ctx.L.fr_grids_create(ctx.h, dim, self.N, self.C, C.byref(h)), "fr_grids_create")
         self.h = h
+        ctx.adopt(self)
 
     @property
     def ptr(self):
@@ -297,6 +311,7 @@ class FFT:
         h = _vp()
         ctx.check(ctx.L.fr_fft_create(ctx.h, dim, int(N), int(batch), int(sign), C.byref(h)), "fr_fft_create")
         self.h = h
+        ctx.adopt(self)
 
     def __call__(self, grids):
         self.ctx.check(self.ctx.L.fr_fft_execute(self.h, grids.h), "fr_fft_execute")
@@ -319,6 +334,7 @@ class DeviceBuffer:
         p = _vp()
         ctx.check(ctx.L.fr_malloc(ctx.h, self.nbytes, C.byref(p)), "fr_malloc")
         self.ptr = p.value
+        ctx.adopt(self)
 
     def to_host(self, dtype, count=None):
         dtype = np.dtype(dtype)
@@ -333,9 +349,9 @@ class DeviceBuffer:
         self.ctx.check(self.ctx.L.fr_memcpy_h2d(self.ctx.h, _vp(self.ptr), _ptr(a), a.nbytes), "fr_memcpy_h2d")
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             self.ctx.L.fr_free(self.ctx.h, _vp(self.ptr))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:

@@ -54,9 +54,15 @@ typedef struct fr_fft fr_fft;
 /* Tuning knobs for a gridding plan; zero fields mean "choose automatically".
  * The LUT length is never tuned in parity mode: it is the L passed in. */
 typedef struct {
-    int tile_x, tile_y, tile_z;  /* grid tile owned by one workgroup        */
-    int slabs;                   /* waves per workgroup; each owns a slab    */
-    int accumulate;              /* 0: LDS add instruction, 1: LDS read-add-write */
+    int tile_x, tile_y, tile_z;  /* base work-unit box (one wave owns a unit) */
+    int slabs;                   /* waves per workgroup                        */
+    int accumulate;              /* 0: LDS read-add-write, 1: LDS float atomic add */
+    int split_taps;              /* heavier units are split (0: automatic)      */
+    int exact_order;             /* 1: split units by space only, so every cell
+                                    sums in ascending sample order (bit-identical
+                                    to the CPU path); 0: heavy units are also split
+                                    by sample range into slices combined in a fixed
+                                    order (deterministic, not bitwise CPU order) */
 } fr_tune;
 
 typedef struct {
@@ -64,6 +70,8 @@ typedef struct {
     int tile_x, tile_y, tile_z, slabs, tiles, lds_bytes;
     int64_t plan_bytes;
     double build_ms;
+    int64_t items, heavy_units, max_unit_taps;
+    int exact_order, pad;
 } fr_plan_info;
 
 /* ------------------------------------------------------------ host helpers */
