@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-recon", action="store_true")
     ap.add_argument("--accumulate", type=int, default=0)
     ap.add_argument("--exact", action="store_true", help="exact-order plan for the timed steps")
+    ap.add_argument("--unit", type=int, nargs=3, default=None, metavar=("X", "Y", "Z"),
+                    help="base work-unit box (default: the plan's choice)")
+    ap.add_argument("--split", type=int, default=0, help="slice/split threshold in taps (0: automatic)")
     ap.add_argument("--no-exact-check", action="store_true")
     return ap.parse_args()
 
@@ -118,7 +121,10 @@ def main():
     torch.cuda.empty_cache()
 
     t0 = time.perf_counter()
-    plan = hip.Plan(ctx, s, N, kw, lut, {"accumulate": a.accumulate, "exact_order": 1 if a.exact else 0})
+    tune = {"accumulate": a.accumulate, "exact_order": 1 if a.exact else 0, "split_taps": a.split}
+    if a.unit:
+        tune.update(tile_x=a.unit[0], tile_y=a.unit[1], tile_z=a.unit[2])
+    plan = hip.Plan(ctx, s, N, kw, lut, tune)
     plan_ms = (time.perf_counter() - t0) * 1e3
     info = plan.info()
     grids = hip.Grids(ctx, 3, N, C)

@@ -61,9 +61,10 @@ __device__ __forceinline__ uint32_t opaque_zero()
     return z;
 }
 
-constexpr int BATCH = 32;          // records per batch
+constexpr int BATCH = 16;          // records per batch
+constexpr int64_t XCD_RUN = 128;   // consecutive items one XCD takes in turn
 constexpr int ACC_STRIDE = FR_ACC_STRIDE;   // floats per accumulator row (64 components + pad)
-constexpr int STAGE = 8;           // 64-tap chunks staged per batch (512 taps)
+constexpr int STAGE = 4;           // 64-tap chunks staged per batch (256 taps)
 
 // One batch of records [base, base + k) with taps from tb; its loads are
 // issued by prep() and consumed by compute() one batch later.
@@ -71,23 +72,24 @@ template <typename TS>
 struct batch_regs {
     float d[BATCH];                // lane p: component p of each record's sample
     float w;                       // lane j: DCF of record j
+    uint32_t nv;                   // lane j: tap count of record j (a multiple of 4)
     uint2 ch[STAGE];               // lane l: tap cb + 64 q + l
-    uint64_t cb, tb;
+    uint32_t cb, tb;
     int k;
 };
 
 template <typename TS>
-__device__ __forceinline__ void prep(batch_regs<TS> &B, uint2 &h, uint64_t &nb, uint64_t &ntb, uint64_t re,
+__device__ __forceinline__ void prep(batch_regs<TS> &B, uint2 &h, uint32_t &nb, uint32_t &ntb, uint32_t re,
                                      const uint2 *__restrict__ H, const uint2 *__restrict__ T,
                                      const TS *__restrict__ data, const float *__restrict__ dcf, int C, int lc,
-                                     uint32_t vz, uint32_t *meta, int lane)
+                                     uint32_t vz, int lane)
 {
     // h holds the headers of records nb .. nb + 63 (lane l: record nb + l)
-    const uint64_t cb = ntb & ~(uint64_t)63;
+    const uint32_t cb = ntb & ~63u;
     int k = 0;
-    uint64_t te = ntb;
+    uint32_t te = ntb;
     for (int j = 0; j < BATCH; ++j) {
-        if (nb + j >= re) break;
+        if (nb + (uint32_t)j >= re) break;
         const uint32_t n = rl(h.y, j);
         if (te + n - cb > 64 * STAGE) break;   // records carry at most 64 taps, so k >= 1
         te += n;
@@ -96,7 +98,7 @@ __device__ __forceinline__ void prep(batch_regs<TS> &B, uint2 &h, uint64_t &nb, 
     B.k = k;
     B.cb = cb;
     B.tb = ntb;
-    meta[lane] = h.y;                          // tap counts of the batch's records
+    B.nv = h.y;                                // tap counts of the batch's records
 #pragma clang loop unroll(full)
     for (int j = 0; j < BATCH; ++j) {          // unconditional: past-the-end records read a valid row
         const uint32_t s = rl(h.x, j < k ? j : 0);
@@ -110,56 +112,40 @@ __device__ __forceinline__ void prep(batch_regs<TS> &B, uint2 &h, uint64_t &nb, 
     h = H[nb + lane];                          // next batch's headers (H is padded)
 }
 
-// Apply the taps of batch B.  A record's taps are distinct cells, so the
-// accumulator words of a group of taps are read together, updated and
-// written back (LDS keeps a wave's accesses in order, so the next group sees
-// these writes).  Tap cells are stored as byte offsets of row cell*ACC_STRIDE.
+// Apply the taps of batch B.  A record's taps are distinct cells (padding
+// taps go to the spare row), so each group's accumulator words are read
+// together, updated and written back; LDS keeps a wave's accesses in order,
+// so the next group and the next record see these writes.  Tap cells are
+// byte offsets of the cell's accumulator row.
 template <int ACC, typename TS>
-__device__ __forceinline__ void compute(const batch_regs<TS> &B, const uint32_t *meta, uint2 *stage, char *accb,
-                                        uint32_t lane4)
+__device__ __forceinline__ void compute(const batch_regs<TS> &B, uint2 *stage, char *accb, uint32_t lane4)
 {
 #pragma clang loop unroll(full)
     for (int q = 0; q < STAGE; ++q) stage[64 * q + (lane4 >> 2)] = B.ch[q];
-    uint32_t off = (uint32_t)(B.tb - B.cb);
+    uint32_t off = B.tb - B.cb;
 #pragma clang loop unroll(full)
     for (int j = 0; j < BATCH; ++j) {
-        const uint32_t n = __builtin_amdgcn_readfirstlane(j < B.k ? meta[j] : 0u);
+        const uint32_t n = j < B.k ? rl(B.nv, j) : 0u;
         const float w = __uint_as_float(rl(__float_as_uint(B.w), j));
         const float d = __fmul_rn(w, B.d[j]);              // dcf * sample (GridLut.cpp:109)
         const uint2 *tp = stage + off;
-        uint32_t q = 0;
-        if constexpr (ACC == 0) {
-            for (; q + 8 <= n; q += 8) {
-                float *p[8];
-                float v[8], x[8];
+        for (uint32_t q = 0; q < n; q += FR_TAP_GROUP) {
+            float *p[FR_TAP_GROUP];
+            float v[FR_TAP_GROUP], x[FR_TAP_GROUP];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const uint2 a = tp[q + u];
-                    p[u] = reinterpret_cast<float *>(accb + (a.x + lane4));
-                    v[u] = __fmul_rn(__uint_as_float(a.y), d);
-                }
+            for (int u = 0; u < FR_TAP_GROUP; ++u) {
+                const uint2 a = tp[q + u];
+                p[u] = reinterpret_cast<float *>(accb + (a.x + lane4));
+                v[u] = __fmul_rn(__uint_as_float(a.y), d);
+            }
+            if constexpr (ACC == 0) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u) x[u] = *p[u];
+                for (int u = 0; u < FR_TAP_GROUP; ++u) x[u] = *p[u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) *p[u] = __fadd_rn(x[u], v[u]);
-            }
-            for (; q + 2 <= n; q += 2) {
-                const uint2 a0 = tp[q], a1 = tp[q + 1];
-                float *p0 = reinterpret_cast<float *>(accb + (a0.x + lane4));
-                float *p1 = reinterpret_cast<float *>(accb + (a1.x + lane4));
-                const float x0 = *p0, x1 = *p1;
-                *p0 = __fadd_rn(x0, __fmul_rn(__uint_as_float(a0.y), d));
-                *p1 = __fadd_rn(x1, __fmul_rn(__uint_as_float(a1.y), d));
-            }
-            if (q < n) {
-                const uint2 a0 = tp[q];
-                float *p0 = reinterpret_cast<float *>(accb + (a0.x + lane4));
-                *p0 = __fadd_rn(*p0, __fmul_rn(__uint_as_float(a0.y), d));
-            }
-        } else {
-            for (; q < n; ++q) {                            // LDS float atomic add (slower on gfx950)
-                const uint2 a0 = tp[q];
-                atomicAdd(reinterpret_cast<float *>(accb + (a0.x + lane4)), __fmul_rn(__uint_as_float(a0.y), d));
+                for (int u = 0; u < FR_TAP_GROUP; ++u) *p[u] = __fadd_rn(x[u], v[u]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < FR_TAP_GROUP; ++u) atomicAdd(p[u], v[u]);   // LDS float atomic (slower)
             }
         }
         off += n;
@@ -173,17 +159,29 @@ __global__ void __launch_bounds__(64) k_grid_items(const fr_item *__restrict__ i
                                                    const TS *__restrict__ data, const float *__restrict__ dcf,
                                                    int C, int stride, int acc_floats, int N, size_t cells,
                                                    float2 *__restrict__ out, float2 *__restrict__ partial,
-                                                   int64_t nslots, int cells_max)
+                                                   int64_t nslots, int cells_max, int64_t nfront)
 {
-    // LDS: accumulator [cells_max][stride] | tap stage [STAGE*64] uint2 | meta [2][64] uint32
+    // LDS: accumulator rows [cells_max + 1][stride] (the last absorbs padding taps) | tap stage
     extern __shared__ float lds[];
     const int lane = threadIdx.x;
-    const int64_t i = blockIdx.x;
+    // Items of the heaviest units run first, one per workgroup.  Workgroups
+    // are dealt to the 8 XCDs round-robin; the remaining (Morton-ordered)
+    // items are remapped so that each XCD takes runs of XCD_RUN consecutive
+    // items, i.e. neighbouring units that share samples in its L2, while the
+    // runs rotate over the XCDs to keep their loads even.
+    const int64_t bid = blockIdx.x;
+    int64_t i;
+    if (bid < nfront) {
+        i = bid;
+    } else {
+        const int64_t r = bid - nfront;
+        const int64_t g = r / 8;
+        const int64_t li = (g / XCD_RUN) * (8 * XCD_RUN) + (bid % 8) * XCD_RUN + g % XCD_RUN;
+        i = nfront + li;
+    }
     if (i >= nitems) return;
     float *acc = lds;
     uint2 *stage = reinterpret_cast<uint2 *>(lds + acc_floats);
-    uint32_t *metaA = reinterpret_cast<uint32_t *>(stage + 64 * STAGE);
-    uint32_t *metaB = metaA + 64;
 
     const fr_item it = items[i];
     const fr_unit U = units[it.unit];
@@ -200,20 +198,20 @@ __global__ void __launch_bounds__(64) k_grid_items(const fr_item *__restrict__ i
     char *accb = reinterpret_cast<char *>(acc);
     const uint32_t lane4 = 4u * lane;
 
-    const uint64_t re = it.r1;
-    uint64_t nb = it.r0, ntb = it.t0;
+    const uint32_t re = it.r1;
+    uint32_t nb = it.r0, ntb = it.t0;
     if (nb < re) {
         // Software pipeline, unrolled twice so buffers A and B never copy:
         // batch B's loads are in flight while batch A is applied.
         uint2 h = H[nb + lane];
         batch_regs<TS> A, B;
-        prep(A, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, metaA, lane);
+        prep(A, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, lane);
         while (true) {
-            prep(B, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, metaB, lane);
-            compute<ACC>(A, metaA, stage, accb, lane4);
+            prep(B, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, lane);
+            compute<ACC>(A, stage, accb, lane4);
             if (B.k == 0) break;
-            prep(A, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, metaA, lane);
-            compute<ACC>(B, metaB, stage, accb, lane4);
+            prep(A, h, nb, ntb, re, H, T, data, dcf, C, lc, vz, lane);
+            compute<ACC>(B, stage, accb, lane4);
             if (A.k == 0) break;
         }
     }
@@ -275,8 +273,9 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
     fr_ctx *ctx = p->ctx;
     const fr_geom &g = p->geom;
     const int stride = ACC_STRIDE;
-    const int acc_floats = g.cells_max * stride;
-    const size_t lds = (size_t)acc_floats * sizeof(float) + sizeof(uint2) * 64 * STAGE + sizeof(uint32_t) * 128;
+    // the tap stage behind the accumulator must be 16-byte aligned (64-bit LDS reads)
+    const int acc_floats = ((g.cells_max + 1) * stride + 3) & ~3;
+    const size_t lds = (size_t)acc_floats * sizeof(float) + sizeof(uint2) * 64 * STAGE;
     const int chunks = (s->C + 31) / 32;
     if (p->nslots > 0) {
         const size_t need = (size_t)chunks * p->nslots * 32 * g.cells_max * sizeof(float2);
@@ -294,10 +293,12 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
     FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (p->nitems == 0) return FR_OK;
     const size_t cells = fr_cells(p->dim, p->N);
-    dim3 grid((unsigned)p->nitems, chunks);
+    const int64_t light = p->nitems - p->nitems_front;
+    const int64_t span = 8 * XCD_RUN;
+    dim3 grid((unsigned)(p->nitems_front + span * ((light + span - 1) / span)), chunks);
     hipLaunchKernelGGL(kern, grid, dim3(64), lds, ctx->stream, p->items, p->nitems, p->units, p->hdrs, p->taps,
                        (const TS *)s->data, s->dcf, s->C, stride, acc_floats, (int)p->N, cells, out,
-                       (float2 *)p->partial, p->nslots, g.cells_max);
+                       (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
     FR_HIP(ctx, hipGetLastError());
     if (p->nheavy > 0) {
         hipLaunchKernelGGL(k_combine, dim3((unsigned)p->nheavy, chunks), dim3(256), 0, ctx->stream, p->heavy,

@@ -45,6 +45,10 @@ struct fr_grids {
 // Accumulator row of one grid cell in LDS: 64 components (32 coils, re/im)
 // plus one pad word so rows start in different banks.
 #define FR_ACC_STRIDE 65
+// Units hold at most 64 cells; the row after a plan's largest unit
+// (row cells_max) absorbs the padding taps that fill records up to a
+// multiple of FR_TAP_GROUP.
+#define FR_TAP_GROUP 4
 
 // A work unit is a box of grid cells owned by one wave.  Its records (one
 // per sample that has taps in the box, ascending sample order) are H[hdr ..
@@ -73,7 +77,8 @@ struct fr_geom {
 struct fr_item {
     uint32_t unit;
     int32_t slot;                // -1: write the grid directly
-    uint64_t r0, r1, t0;
+    uint32_t r0, r1, t0;         // records [r0, r1), first tap t0
+    uint32_t pad;
 };
 
 // A heavy unit: its items are [item0, item0 + nitems), partial slots
@@ -105,6 +110,7 @@ struct fr_plan {
     int exact_order = 0;               // 1: no sample slicing (bitwise CPU order)
     fr_item *items = nullptr;          // device, launch order
     int64_t nitems = 0;
+    int64_t nitems_front = 0;          // items launched first, one per workgroup
     fr_heavy *heavy = nullptr;         // device
     int64_t nheavy = 0, nslots = 0;
     float *partial = nullptr;          // scratch for heavy-unit slices

@@ -114,10 +114,18 @@ __global__ void k_emit_records(const float *__restrict__ traj, int64_t M, gctx g
         });
 }
 
+// Records are padded to a multiple of FR_TAP_GROUP taps with dummy taps
+// (the spare accumulator row, weight 0), so the kernel applies taps in
+// fixed-size groups without remainder handling.
+__host__ __device__ __forceinline__ uint32_t padded_taps(uint32_t n)
+{
+    return (n + FR_TAP_GROUP - 1) / FR_TAP_GROUP * FR_TAP_GROUP;
+}
+
 __global__ void k_record_taps(const uint64_t *__restrict__ vals, int64_t R, uint64_t *__restrict__ n)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R) n[r] = vals[r] & 0xffffffffu;
+    if (r < R) n[r] = padded_taps((uint32_t)(vals[r] & 0xffffffffu));
 }
 
 __global__ void k_unit_ranges(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ toff, int64_t R,
@@ -154,14 +162,15 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
                                const int32_t *__restrict__ cell_unit, const fr_unit *__restrict__ units,
                                const uint32_t *__restrict__ keys, const uint64_t *__restrict__ vals,
                                const uint64_t *__restrict__ toff, int64_t R, const float *__restrict__ lut,
-                               uint2 *__restrict__ H, uint2 *__restrict__ T)
+                               uint32_t dummy_row, uint2 *__restrict__ H, uint2 *__restrict__ T)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const uint32_t key = keys[r];
     const int64_t s = (int64_t)(vals[r] >> 32);
     const uint32_t n = (uint32_t)(vals[r] & 0xffffffffu);
-    H[r] = make_uint2((uint32_t)s, n);
+    const uint32_t np = padded_taps(n);
+    H[r] = make_uint2((uint32_t)s, np);
     fr_box b;
     fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
     uint2 *w = T + toff[r];
@@ -174,6 +183,7 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
             w[i++] = make_uint2(cell * (4u * FR_ACC_STRIDE), __float_as_uint(lut[ki]));
         });
     });
+    for (; i < np; ++i) w[i] = make_uint2(dummy_row * (4u * FR_ACC_STRIDE), 0u);
 }
 
 // A record opens a new work item when it is its unit's first record or when
@@ -203,7 +213,7 @@ __global__ void k_make_items(const uint32_t *__restrict__ keys, const uint64_t *
     uint64_t r1 = U.hdr + U.nrec;
     for (uint64_t q = (uint64_t)r + 1; q < U.hdr + U.nrec; ++q)
         if (flag[q]) { r1 = q; break; }
-    items[i] = fr_item{u, -1, (uint64_t)r, r1, toff[r]};
+    items[i] = fr_item{u, -1, (uint32_t)r, (uint32_t)r1, (uint32_t)toff[r]};
     if ((uint64_t)r == U.hdr) first_item[u] = i;
     if (r1 == U.hdr + U.nrec) last_item[u] = i;
 }
@@ -214,7 +224,7 @@ __global__ void k_empty_items(const fr_unit *__restrict__ units, int64_t nunits,
 {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits || units[u].nrec != 0) return;
-    items[base + eidx[u] - 1] = fr_item{(uint32_t)u, -1, 0, 0, 0};
+    items[base + eidx[u] - 1] = fr_item{(uint32_t)u, -1, 0u, 0u, 0u};
 }
 
 __global__ void k_empty_flags(const fr_unit *__restrict__ units, int64_t nunits, uint32_t *__restrict__ f)
@@ -263,18 +273,20 @@ int floor_pow2(int v)
 
 }  // namespace
 
-// Base unit box: 64 cells (8 x 8 in a plane), x fastest for coalesced
-// write-out.  A unit never exceeds 64 cells, so a record (one sample's taps
-// inside one unit) never carries more than 64 taps.
+// Base unit box: 32 cells (4 x 4 x 2 in 3D, 8 x 4 in 2D).  A unit never
+// exceeds 64 cells, so a record (one sample's taps inside one unit) never
+// carries more than 64 taps.  Smaller units cost more records but use less
+// LDS per wave, so more waves per CU hide the LDS and memory latency; on
+// MI355X 16-32 cells per unit measured best for the 32-coil kooshball.
 static fr_geom choose_geom(int dim, unsigned N, int coils, const fr_tune *tune)
 {
     (void)coils;
     fr_geom g{};
     g.dim = dim;
     g.N = (int)N;
-    g.bx = 8;
-    g.by = 8;
-    g.bz = 1;
+    g.bx = dim == 3 ? 4 : 8;
+    g.by = 4;
+    g.bz = dim == 3 ? 2 : 1;
     g.waves = 1;
     if (tune) {
         if (tune->tile_x > 0) g.bx = tune->tile_x;
@@ -388,7 +400,26 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
                 ub.split(b);
             }
     std::vector<host_unit> &hu = ub.out;
-    std::stable_sort(hu.begin(), hu.end(), [](const host_unit &a, const host_unit &b) { return a.taps > b.taps; });
+    // Launch order: units heavier than a quarter of the split threshold first
+    // (longest first), then the rest along a Morton curve of their origins so
+    // that items running at the same time share samples in cache.
+    const uint64_t heavy_cut = thr / 4;
+    auto morton = [&](const host_unit &u) {
+        const uint32_t a = (uint32_t)u.ox / geom.bx, b = (uint32_t)u.oy / geom.by, c = (uint32_t)u.oz / geom.bz;
+        uint64_t m = 0;
+        for (int bit = 0; bit < 21; ++bit)
+            m |= ((uint64_t)((a >> bit) & 1) << (3 * bit)) | ((uint64_t)((b >> bit) & 1) << (3 * bit + 1)) |
+                 ((uint64_t)((c >> bit) & 1) << (3 * bit + 2));
+        return m;
+    };
+    std::stable_sort(hu.begin(), hu.end(), [&](const host_unit &a, const host_unit &b) {
+        const bool ha = a.taps > heavy_cut, hb = b.taps > heavy_cut;
+        if (ha != hb) return ha;
+        if (ha) return a.taps > b.taps;
+        return morton(a) < morton(b);
+    });
+    int64_t nheavy_units = 0;
+    while (nheavy_units < (int64_t)hu.size() && hu[nheavy_units].taps > heavy_cut) ++nheavy_units;
     const int64_t nunits = (int64_t)hu.size();
     std::vector<fr_unit> units(nunits);
     std::vector<int32_t> cell_unit(cells);
@@ -460,6 +491,8 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     uint64_t TT = 0;
     FR_HIP(ctx, hipMemcpyAsync(&TT, toff.p + R, 8, hipMemcpyDeviceToHost, st));
     FR_HIP(ctx, hipStreamSynchronize(st));
+    FR_CHECK(ctx, TT + 1024 < (uint64_t(1) << 32), FR_ERR_UNSUPPORTED, "plan has %llu taps (max 2^32 - 1024)",
+             (unsigned long long)TT);
 
     // 5. unit record/tap ranges, then the records themselves
     if ((e = fr_alloc(ctx, (void **)&p->hdrs, sizeof(uint2) * (R + 64)))) return e;
@@ -472,7 +505,8 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
                            p->units);
         hipLaunchKernelGGL(k_unit_counts, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, toff.p);
         hipLaunchKernelGGL(k_fill_records, dim3(nblk(R, 128)), dim3(128), 0, st, s->traj, M, g, cu_d.p, p->units,
-                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, p->hdrs, p->taps);
+                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, (uint32_t)geom.cells_max, p->hdrs,
+                           p->taps);
     }
     FR_HIP(ctx, hipGetLastError());
 
@@ -503,6 +537,7 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
         FR_HIP(ctx, hipMemcpyAsync(&nempty, eidx.p + nunits - 1, 4, hipMemcpyDeviceToHost, st));
         FR_HIP(ctx, hipStreamSynchronize(st));
         p->nitems = (int64_t)nrec_items + nempty;
+
         if ((e = fr_alloc(ctx, (void **)&p->items, sizeof(fr_item) * std::max<int64_t>(1, p->nitems)))) return e;
         FR_HIP(ctx, hipMemsetAsync(first.p, 0, 4 * nu, st));
         FR_HIP(ctx, hipMemsetAsync(last.p, 0, 4 * nu, st));
@@ -525,6 +560,19 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
                            sscan.p, first.p, p->heavy, p->items);
         FR_HIP(ctx, hipGetLastError());
         FR_HIP(ctx, hipStreamSynchronize(st));
+        // items of the units ordered first (the heavy ones) are launched in
+        // order one per workgroup; the remaining items are spread over the
+        // XCDs in contiguous ranges (see fr_grid.hip)
+        p->nitems_front = 0;
+        if (nheavy_units > 0 && nrec_items > 0) {
+            std::vector<fr_item> hitems(nrec_items);
+            FR_HIP(ctx, hipMemcpyAsync(hitems.data(), p->items, sizeof(fr_item) * nrec_items,
+                                       hipMemcpyDeviceToHost, st));
+            FR_HIP(ctx, hipStreamSynchronize(st));
+            int64_t f = 0;
+            while (f < (int64_t)nrec_items && (int64_t)hitems[f].unit < nheavy_units) ++f;
+            p->nitems_front = f;
+        }
     }
     p->nrecords = (int64_t)R;
     p->ntaps = (int64_t)TT;
