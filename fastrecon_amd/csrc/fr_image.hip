@@ -185,3 +185,170 @@ extern "C" fr_status fr_deapod_map(fr_ctx *ctx, int dim, unsigned N, float kWidt
     fr_samples_destroy(s);
     return st;
 }
+
+// ------------------------------------------------------- PILS / crop ops
+namespace {
+
+// ImageFilter::lowFilter (ImageFilter.cpp:55-93): x0 = size/2 per axis,
+// r = squared integer distance, idx = int(float(r) / att * 100.0) in
+// double, zero past the 2000-entry table, else scale by coeff[idx].
+__global__ void k_lowfilter(float2 *__restrict__ g, int dim, int N, int C, size_t cells, float att,
+                            const float *__restrict__ coeff)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * C) return;
+    const size_t e = i % cells;
+    const int x = (int)(e % N), y = (int)((e / N) % N), z = dim == 3 ? (int)(e / ((size_t)N * N)) : 0;
+    const int h = N / 2, hz = dim == 3 ? N / 2 : 0;   // z0 = size.z / 2 with size.z = 1 in 2D
+    const int r = (z - hz) * (z - hz) + (y - h) * (y - h) + (x - h) * (x - h);
+    const int idx = (int)((double)__fdiv_rn((float)r, att) * 100.0);
+    float2 v = g[i];
+    if (idx >= 2000) {
+        v = make_float2(0.f, 0.f);
+    } else {
+        const float c = coeff[idx];
+        v = make_float2(__fmul_rn(v.x, c), __fmul_rn(v.y, c));
+    }
+    g[i] = v;
+}
+
+// ImageFilter::normalize (ImageFilter.cpp:95-118): v /= sqrtf(sum_c norm(v_c)).
+// libstdc++'s std::norm(complex<float>) is abs(v)^2 with abs = cabsf, which
+// glibc evaluates as the float rounding of the double sqrt(re^2 + im^2); the
+// same is done here, then squared in float.
+__global__ void k_normalize(float2 *__restrict__ g, int C, size_t cells)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    float m = 0.f;
+    for (int c = 0; c < C; ++c) {
+        const float2 v = g[(size_t)c * cells + i];
+        const double re = v.x, im = v.y;
+        const float a = (float)__dsqrt_rn(__dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im)));
+        m = __fadd_rn(m, __fmul_rn(a, a));
+    }
+    const float s = sqrtf(m);
+    for (int c = 0; c < C; ++c) {
+        float2 v = g[(size_t)c * cells + i];
+        g[(size_t)c * cells + i] = make_float2(__fdiv_rn(v.x, s), __fdiv_rn(v.y, s));
+    }
+}
+
+// ImageFilter::crop (ImageFilter.cpp:120-157): channel-wise centre crop
+__global__ void k_crop(const float2 *__restrict__ in, int dim, int N, int C, size_t cells, int rcx, int rcy, int rcz,
+                       float2 *__restrict__ out)
+{
+    const size_t n = (size_t)rcx * rcy * rcz;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * C) return;
+    const int c = (int)(i / n);
+    const size_t o = i - (size_t)c * n;
+    const int x = (int)(o % rcx), y = (int)((o / rcx) % rcy), z = (int)(o / ((size_t)rcx * rcy));
+    const int nz = dim == 3 ? N : 1;
+    const size_t src = ((size_t)((nz - rcz) / 2 + z) * N + (N - rcy) / 2 + y) * N + (N - rcx) / 2 + x;
+    out[i] = in[(size_t)c * cells + src];
+}
+
+// ImageFilter::SOS(map, size) (ImageFilter.cpp:170-209): out = sum over
+// coils in order of v * conj(m) on the crop, complex FP32 (no sqrt).
+__global__ void k_sos_map(const float2 *__restrict__ g, const float2 *__restrict__ m, int dim, int N, int C,
+                          size_t cells, int rcx, int rcy, int rcz, int shift, float2 *__restrict__ out)
+{
+    const size_t n = (size_t)rcx * rcy * rcz;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % rcx), y = (int)((i / rcx) % rcy), z = (int)(i / ((size_t)rcx * rcy));
+    const int nz = dim == 3 ? N : 1;
+    int sx = (N - rcx) / 2 + x, sy = (N - rcy) / 2 + y, sz = (nz - rcz) / 2 + z;
+    if (shift) {
+        const int h = N / 2;
+        sx = sx < h ? sx + h : sx - h;
+        sy = sy < h ? sy + h : sy - h;
+        if (dim == 3) sz = sz < h ? sz + h : sz - h;
+    }
+    const size_t src = ((size_t)sz * N + sy) * N + sx;
+    float ar = 0.f, ai = 0.f;
+    for (int c = 0; c < C; ++c) {
+        const float2 v = g[(size_t)c * cells + src];
+        const float2 w = m[(size_t)c * cells + src];
+        // (vr + i vi) * (wr - i wi): real vr*wr - vi*(-wi), imag vr*(-wi) + vi*wr
+        const float re = __fsub_rn(__fmul_rn(v.x, w.x), __fmul_rn(v.y, -w.y));
+        const float im = __fadd_rn(__fmul_rn(v.x, -w.y), __fmul_rn(v.y, w.x));
+        ar = __fadd_rn(ar, re);
+        ai = __fadd_rn(ai, im);
+    }
+    out[i] = make_float2(ar, ai);
+}
+
+}  // namespace
+
+extern "C" fr_status fr_lowfilter(fr_ctx *ctx, fr_grids *g, int res)
+{
+    if (!ctx || !g) return FR_ERR_INVALID;
+    FR_CHECK(ctx, res > 0, FR_ERR_INVALID, "lowFilter resolution must be positive");
+    // coefficients exactly as the reference's host loop: expf(-r / 100.0)
+    float coeff[2000];
+    for (int r = 0; r < 2000; ++r) coeff[r] = expf((float)(-r / 100.0));
+    const float att = (float)(2.0 * res * res / 4.0);
+    float *dc = nullptr;
+    fr_status st = fr_alloc(ctx, (void **)&dc, sizeof coeff);
+    if (st) return st;
+    hipError_t e = hipMemcpyAsync(dc, coeff, sizeof coeff, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        const size_t n = g->cells * g->C;
+        hipLaunchKernelGGL(k_lowfilter, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N,
+                           g->C, g->cells, att, dc);
+        e = hipStreamSynchronize(ctx->stream);
+    }
+    hipFree(dc);
+    FR_CHECK(ctx, e == hipSuccess, FR_ERR_HIP, "lowFilter: %s", hipGetErrorString(e));
+    return FR_OK;
+}
+
+extern "C" fr_status fr_normalize(fr_ctx *ctx, fr_grids *g)
+{
+    if (!ctx || !g) return FR_ERR_INVALID;
+    hipLaunchKernelGGL(k_normalize, dim3(nblk(g->cells, 256)), dim3(256), 0, ctx->stream, g->data, g->C, g->cells);
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_crop(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz, void *out_dev)
+{
+    if (!ctx || !g || !out_dev) return FR_ERR_INVALID;
+    if (g->dim == 2) rcz = 1;
+    FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
+             FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
+    const size_t n = (size_t)rcx * rcy * rcz * g->C;
+    hipLaunchKernelGGL(k_crop, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N, g->C,
+                       g->cells, (int)rcx, (int)rcy, (int)rcz, (float2 *)out_dev);
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_sos_map(fr_ctx *ctx, const fr_grids *g, const fr_grids *map, unsigned rcx, unsigned rcy,
+                                unsigned rcz, int apply_shift, void *out_dev)
+{
+    if (!ctx || !g || !map || !out_dev) return FR_ERR_INVALID;
+    if (g->dim == 2) rcz = 1;
+    FR_CHECK(ctx, map->dim == g->dim && map->N == g->N && map->C >= g->C, FR_ERR_INVALID,
+             "map grids do not match the image grids");
+    FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
+             FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
+    FR_CHECK(ctx, !apply_shift || g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size");
+    const size_t n = (size_t)rcx * rcy * rcz;
+    hipLaunchKernelGGL(k_sos_map, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, map->data, g->dim,
+                       (int)g->N, g->C, g->cells, (int)rcx, (int)rcy, (int)rcz, apply_shift, (float2 *)out_dev);
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_grids_copy(fr_ctx *ctx, const fr_grids *src, fr_grids *dst)
+{
+    if (!ctx || !src || !dst) return FR_ERR_INVALID;
+    FR_CHECK(ctx, src->dim == dst->dim && src->N == dst->N && src->C == dst->C, FR_ERR_INVALID,
+             "grid copy between different shapes");
+    FR_HIP(ctx, hipMemcpyAsync(dst->data, src->data, sizeof(float2) * src->cells * src->C, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    return FR_OK;
+}

@@ -27,7 +27,8 @@ EXPORTS = [
     "fr_grids_create", "fr_grids_destroy", "fr_grids_download", "fr_grids_upload", "fr_grids_device_ptr",
     "fr_grid_execute", "fr_degrid_execute",
     "fr_fft_create", "fr_fft_destroy", "fr_fft_execute",
-    "fr_fftshift", "fr_sos", "fr_deapod_map",
+    "fr_fftshift", "fr_sos", "fr_deapod_map", "fr_lowfilter", "fr_normalize", "fr_crop", "fr_sos_map",
+    "fr_grids_copy",
     "fr_dcf_pipe_menon",
     "fr_malloc", "fr_free", "fr_memcpy_d2h", "fr_memcpy_h2d",
 ]
@@ -97,6 +98,11 @@ def load():
         "fr_sos": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp],
         "fr_deapod_map": [_vp, C.c_int, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
         "fr_dcf_pipe_menon": [_vp, _vp, C.c_int, C.c_int],
+        "fr_lowfilter": [_vp, _vp, C.c_int],
+        "fr_normalize": [_vp, _vp],
+        "fr_crop": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
+        "fr_sos_map": [_vp, _vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp],
+        "fr_grids_copy": [_vp, _vp, _vp],
         "fr_malloc": [_vp, C.c_size_t, _pp],
         "fr_free": [_vp, _vp],
         "fr_memcpy_d2h": [_vp, _vp, _vp, C.c_size_t],
@@ -400,3 +406,34 @@ def deapod_map(ctx, dim, N, kwidth, lut, centre, rc):
 def pipe_menon(plan, samples, iters=10, init_ones=True):
     plan.ctx.check(plan.ctx.L.fr_dcf_pipe_menon(plan.h, samples.h, iters, 1 if init_ones else 0),
                    "fr_dcf_pipe_menon")
+
+
+def lowfilter(ctx, grids, res):
+    ctx.check(ctx.L.fr_lowfilter(ctx.h, grids.h, int(res)), "fr_lowfilter")
+
+
+def normalize(ctx, grids):
+    ctx.check(ctx.L.fr_normalize(ctx.h, grids.h), "fr_normalize")
+
+
+def crop(ctx, grids, rc):
+    rcx, rcy, rcz = rc
+    if grids.dim == 2:
+        rcz = 1
+    out = DeviceBuffer(ctx, 8 * rcx * rcy * rcz * grids.C)
+    ctx.check(ctx.L.fr_crop(ctx.h, grids.h, rcx, rcy, rcz, _vp(out.ptr)), "fr_crop")
+    return out
+
+
+def sos_map(ctx, grids, cmap, rc, shift=True):
+    rcx, rcy, rcz = rc
+    if grids.dim == 2:
+        rcz = 1
+    out = DeviceBuffer(ctx, 8 * rcx * rcy * rcz)
+    ctx.check(ctx.L.fr_sos_map(ctx.h, grids.h, cmap.h, rcx, rcy, rcz, 1 if shift else 0, _vp(out.ptr)),
+              "fr_sos_map")
+    return out
+
+
+def grids_copy(ctx, src, dst):
+    ctx.check(ctx.L.fr_grids_copy(ctx.h, src.h, dst.h), "fr_grids_copy")

@@ -171,6 +171,23 @@ fr_status fr_fftshift(fr_ctx *ctx, fr_grids *g);
  * read (the grids are left unshifted).  deapod_dev may be NULL. */
 fr_status fr_sos(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz,
                  int apply_shift, const float *deapod_dev, float *out_dev);
+/* ImageFilter::lowFilter (ImageFilter.cpp:55-93), in place on all coils;
+ * the 2000-entry expf table is evaluated on the host exactly as the
+ * reference does. */
+fr_status fr_lowfilter(fr_ctx *ctx, fr_grids *g, int res);
+/* ImageFilter::normalize (ImageFilter.cpp:95-118): v /= sqrtf(sum_c |v_c|^2). */
+fr_status fr_normalize(fr_ctx *ctx, fr_grids *g);
+/* ImageFilter::crop (ImageFilter.cpp:120-157): out_dev = C crops of
+ * rcx*rcy*rcz complex64 (coil-major). */
+fr_status fr_crop(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz, void *out_dev);
+/* ImageFilter::SOS(map, size) (ImageFilter.cpp:170-209), the PILS combine:
+ * out_dev[rc] = sum_c v_c * conj(map_c) (complex64, no square root);
+ * apply_shift folds fftShift into the reads of both inputs. */
+fr_status fr_sos_map(fr_ctx *ctx, const fr_grids *g, const fr_grids *map, unsigned rcx, unsigned rcy, unsigned rcz,
+                     int apply_shift, void *out_dev);
+/* Device copy of all coils between grids of the same shape (ImageData copy,
+ * ImageData.cpp:83-95). */
+fr_status fr_grids_copy(fr_ctx *ctx, const fr_grids *src, fr_grids *dst);
 /* Deapodisation map (net-new): |FFT| of one unit sample gridded at the
  * normalised position `centre` with the same LUT, shifted and cropped.
  * out_dev: rcx*rcy*rcz floats. */

@@ -458,3 +458,88 @@ void or_pipe_menon_f64(int dim, unsigned N, int64_t M, const float *traj, int st
     }
     free(g); free(d);
 }
+
+/* ImageFilter::lowFilter (ImageFilter.cpp:56-93). */
+void or_lowfilter_f32(int dim, unsigned N, int C, float *data, int res)
+{
+    unsigned nx = N, ny = N, nz = (dim == 3) ? N : 1u;
+    unsigned x0 = nx / 2u, y0 = ny / 2u, z0 = nz / 2u;
+    float att = (float)(2.0 * res * res / 4.0);
+    float coeff[2000];
+    for (int r = 0; r < 2000; ++r) coeff[r] = expf((float)(-r / 100.0));
+    size_t cells = (size_t)nx * ny * nz;
+    for (int c = 0; c < C; ++c) {
+        float *d = data + 2 * cells * (size_t)c;
+        size_t i = 0;
+        for (unsigned long z = 0; z < nz; ++z) {
+            int r1 = (int)((z - z0) * (z - z0));
+            for (unsigned long y = 0; y < ny; ++y) {
+                int r2 = (int)((y - y0) * (y - y0)) + r1;
+                for (unsigned long x = 0; x < nx; ++x, ++i) {
+                    int r = (int)((x - x0) * (x - x0) + (unsigned long)r2);
+                    int idx = (int)((double)((float)r / att) * 100.0);
+                    if (idx >= 2000) {
+                        d[2 * i] = 0.0f;
+                        d[2 * i + 1] = 0.0f;
+                    } else {
+                        d[2 * i] *= coeff[idx];
+                        d[2 * i + 1] *= coeff[idx];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ImageFilter::normalize (ImageFilter.cpp:95-118). */
+void or_normalize_f32(int dim, unsigned N, int C, float *data)
+{
+    size_t n = N;
+    size_t cells = (dim == 3) ? n * n * n : n * n;
+    float *mag = (float *)calloc(cells, sizeof(float));
+    for (int c = 0; c < C; ++c) {
+        const float *d = data + 2 * cells * (size_t)c;
+        for (size_t i = 0; i < cells; ++i) {
+            float a = hypotf(d[2 * i], d[2 * i + 1]);
+            mag[i] = mag[i] + a * a;
+        }
+    }
+    for (int c = 0; c < C; ++c) {
+        float *d = data + 2 * cells * (size_t)c;
+        for (size_t i = 0; i < cells; ++i) {
+            float s = sqrtf(mag[i]);
+            d[2 * i] = d[2 * i] / s;
+            d[2 * i + 1] = d[2 * i + 1] / s;
+        }
+    }
+    free(mag);
+}
+
+/* ImageFilter::SOS(map, size) (ImageFilter.cpp:170-209). */
+void or_sos_map_f32(int dim, unsigned N, int C, const float *in, const float *map,
+                    unsigned rcx, unsigned rcy, unsigned rcz, float *out)
+{
+    size_t n = N;
+    size_t cells = (dim == 3) ? n * n * n : n * n;
+    unsigned nz = (dim == 3) ? N : 1u;
+    if (dim == 2) rcz = 1u;
+    unsigned x0 = (N - rcx) / 2u, y0 = (N - rcy) / 2u, z0 = (nz - rcz) / 2u;
+    size_t rc = (size_t)rcx * rcy * rcz;
+    for (size_t i = 0; i < 2 * rc; ++i) out[i] = 0.0f;
+    for (int c = 0; c < C; ++c) {
+        const float *d = in + 2 * cells * (size_t)c;
+        const float *m = map + 2 * cells * (size_t)c;
+        for (unsigned z = 0; z < rcz; ++z)
+            for (unsigned y = 0; y < rcy; ++y)
+                for (unsigned x = 0; x < rcx; ++x) {
+                    size_t i = (size_t)(z + z0) * n * n + (size_t)(y + y0) * n + (x + x0);
+                    float a = d[2 * i], b = d[2 * i + 1];
+                    float mc = m[2 * i], md = -m[2 * i + 1];
+                    float re = a * mc - b * md;
+                    float im = a * md + b * mc;
+                    size_t o = ((size_t)z * rcy + y) * rcx + x;
+                    out[2 * o] = out[2 * o] + re;
+                    out[2 * o + 1] = out[2 * o + 1] + im;
+                }
+    }
+}

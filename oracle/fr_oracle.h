@@ -22,6 +22,11 @@
  *   or_fft_c2c       -- numpy.fft (independent library), Parseval   (pinned)
  *   or_fftshift      -- numpy.fft.fftshift, involution              (pinned)
  *   or_sos           -- closed form in numpy                        (pinned)
+ *   or_lowfilter     -- float64 exp(-floor(r/att*100)/100) window,
+ *                       centre unchanged, zero beyond the cut-off    (pinned)
+ *   or_normalize     -- unit coil norm per voxel, phase ratios kept (pinned)
+ *   or_sos_map       -- map == data gives SOS^2 (real), conj-linear
+ *                       in the map                                 (pinned)
  *   or_degrid_f64    -- adjoint identity against or_grid_f64        (pinned)
  */
 #ifndef FR_ORACLE_H
@@ -88,6 +93,20 @@ void or_sos_f32(int dim, unsigned N, int C, const float *in,
                 unsigned rcx, unsigned rcy, unsigned rcz, float *out);
 void or_sos_f64(int dim, unsigned N, int C, const double *in,
                 unsigned rcx, unsigned rcy, unsigned rcz, double *out);
+
+/* ImageFilter::lowFilter (ImageFilter.cpp:56-93): Gaussian-like k-space
+ * window exp(-idx/100), idx = (int)((float)r / att * 100.0),
+ * att = (float)(2.0*res*res/4.0), r = |(x,y,z) - size/2|^2 (integer),
+ * zero where idx >= 2000.  In place on C x N^dim complex64. */
+void or_lowfilter_f32(int dim, unsigned N, int C, float *data, int res);
+/* ImageFilter::normalize (ImageFilter.cpp:95-118): per voxel
+ * mag = sum_c std::norm(v_c) (libstdc++: fl(cabsf(v)^2)), v_c /= sqrtf(mag). */
+void or_normalize_f32(int dim, unsigned N, int C, float *data);
+/* ImageFilter::SOS(map, size) (ImageFilter.cpp:170-209), the PILS combine:
+ * out[rc] = sum_c v_c * conj(map_c) over the centred crop, FP32 complex
+ * arithmetic in coil order.  out: rc complex64. */
+void or_sos_map_f32(int dim, unsigned N, int C, const float *in, const float *map,
+                    unsigned rcx, unsigned rcy, unsigned rcz, float *out);
 
 /* Pipe-Menon iteration w <- w / G^H G w on the oracle's own FP64 gridding
  * (single channel, real weights).  Not in the reference (net-new). */

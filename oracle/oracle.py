@@ -48,6 +48,10 @@ def lib():
         L.or_sos_f64.argtypes = [c.c_int, c.c_uint, c.c_int, _f64p, c.c_uint, c.c_uint, c.c_uint, _f64p]
         L.or_pipe_menon_f64.argtypes = [c.c_int, c.c_uint, c.c_int64, _f32p, c.c_int, c.c_float,
                                         _f32p, c.c_int, c.c_int, _f64p]
+        L.or_lowfilter_f32.argtypes = [c.c_int, c.c_uint, c.c_int, _f32p, c.c_int]
+        L.or_normalize_f32.argtypes = [c.c_int, c.c_uint, c.c_int, _f32p]
+        L.or_sos_map_f32.argtypes = [c.c_int, c.c_uint, c.c_int, _f32p, _f32p, c.c_uint, c.c_uint,
+                                     c.c_uint, _f32p]
         _lib = L
     return _lib
 
@@ -174,3 +178,43 @@ def recon_f32(traj_norm, dcf, data, N, kwidth, lut, rc, nthreads=1):
     f = fft_c2c(dim, N, g, +1).astype(np.complex64)
     f = fftshift(dim, N, f)
     return sos(dim, N, f, rc)
+
+
+def lowfilter(dim, N, data, res):
+    """ImageFilter::lowFilter (ImageFilter.cpp:56-93) on C x N^dim complex64."""
+    a = np.ascontiguousarray(data, dtype=np.complex64).copy()
+    lib().or_lowfilter_f32(dim, N, a.shape[0], a.view(np.float32).reshape(-1), res)
+    return a
+
+
+def normalize(dim, N, data):
+    """ImageFilter::normalize (ImageFilter.cpp:95-118)."""
+    a = np.ascontiguousarray(data, dtype=np.complex64).copy()
+    lib().or_normalize_f32(dim, N, a.shape[0], a.view(np.float32).reshape(-1))
+    return a
+
+
+def sos_map(dim, N, data, cmap, rc):
+    """ImageFilter::SOS(map, size) (ImageFilter.cpp:170-209): complex64 crop."""
+    rcx, rcy, rcz = rc
+    if dim == 2:
+        rcz = 1
+    a = np.ascontiguousarray(data, dtype=np.complex64)
+    m = np.ascontiguousarray(cmap, dtype=np.complex64)
+    out = np.zeros(rcx * rcy * rcz, np.complex64)
+    lib().or_sos_map_f32(dim, N, a.shape[0], a.view(np.float32).reshape(-1),
+                         m.view(np.float32).reshape(-1), rcx, rcy, rcz, out.view(np.float32))
+    return out
+
+
+def pils_f32(traj_norm, dcf, data, N, kwidth, lut, rc, nthreads=1):
+    """Reference PILS pipeline (Main.cpp:116-170): the coil map is the
+    gridded k-space low-pass filtered (res 22), FFT'd, shifted and
+    normalised; the image is sum_c img_c conj(map_c); the file holds |.|."""
+    M, dim = np.asarray(traj_norm).shape
+    g = grid_f32(traj_norm, dcf, data, N, kwidth, lut, nthreads)
+    img = fftshift(dim, N, fft_c2c(dim, N, g, +1).astype(np.complex64))
+    m = lowfilter(dim, N, g, 22)
+    m = fftshift(dim, N, fft_c2c(dim, N, m, +1).astype(np.complex64))
+    m = normalize(dim, N, m)
+    return np.abs(sos_map(dim, N, img, m, rc)).astype(np.float32)

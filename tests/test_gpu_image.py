@@ -121,3 +121,49 @@ def test_pipe_menon_matches_oracle(ctx, orc, dim):
     e = rel_l2(got, ref)
     print(f"[pipe-menon dim={dim}] 10 iterations, FP32 GPU vs FP64 oracle rel-L2 = {e:.2e}", flush=True)
     assert e <= 1e-6          # ten FP32 iterations of a ratio update (measured ~1.5e-7)
+
+
+@pytest.mark.parametrize("dim,N,C", [(2, 64, 4), (3, 32, 3)])
+def test_pils_ops_bitwise(ctx, orc, dim, N, C):
+    """lowFilter, normalize, crop and SOS(map) are element-wise FP32 with the
+    reference's operation order: bit-identical to the oracle."""
+    rng = np.random.default_rng(17)
+    cells = N ** dim
+    d = (rng.standard_normal((C, cells)) + 1j * rng.standard_normal((C, cells))).astype(np.complex64)
+    m = (rng.standard_normal((C, cells)) + 1j * rng.standard_normal((C, cells))).astype(np.complex64)
+    g = hip.Grids(ctx, dim, N, C)
+    for c in range(C):
+        g.upload(c, d[c])
+    hip.lowfilter(ctx, g, 7)
+    assert np.array_equal(g.download_all(), orc.lowfilter(dim, N, d, 7))
+    for c in range(C):
+        g.upload(c, d[c])
+    hip.normalize(ctx, g)
+    assert np.array_equal(g.download_all(), orc.normalize(dim, N, d))
+
+    rc = (N // 2, N // 2 + 2, N // 2 if dim == 3 else 1)
+    for c in range(C):
+        g.upload(c, d[c])
+    out = hip.crop(ctx, g, rc).to_host(np.complex64).reshape(C, -1)
+    full = d.reshape((C,) + (N,) * dim)
+    x0, y0 = (N - rc[0]) // 2, (N - rc[1]) // 2
+    if dim == 3:
+        z0 = (N - rc[2]) // 2
+        ref = full[:, z0:z0 + rc[2], y0:y0 + rc[1], x0:x0 + rc[0]]
+    else:
+        ref = full[:, y0:y0 + rc[1], x0:x0 + rc[0]]
+    assert np.array_equal(out, ref.reshape(C, -1))
+
+    gm = hip.Grids(ctx, dim, N, C)
+    for c in range(C):
+        gm.upload(c, m[c])
+    got = hip.sos_map(ctx, g, gm, rc, shift=False).to_host(np.complex64)
+    assert np.array_equal(got, orc.sos_map(dim, N, d, m, rc))
+    # fftshift folded into the reads == explicit shift of both inputs
+    got_s = hip.sos_map(ctx, g, gm, rc, shift=True).to_host(np.complex64)
+    ref_s = orc.sos_map(dim, N, orc.fftshift(dim, N, d), orc.fftshift(dim, N, m), rc)
+    assert np.array_equal(got_s, ref_s)
+    # grids_copy is a device copy of every coil
+    g2 = hip.Grids(ctx, dim, N, C)
+    hip.grids_copy(ctx, g, g2)
+    assert np.array_equal(g2.download_all(), d)

@@ -57,3 +57,80 @@ def test_sos_closed_form(orc, dim, N, rc):
     np.testing.assert_allclose(got, ref, rtol=2e-7)
     got64 = orc.sos(dim, N, x.astype(np.complex128), rc)
     np.testing.assert_allclose(got64, ref, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim,N,res", [(2, 64, 22), (3, 32, 22), (2, 48, 5)])
+def test_lowfilter_window(orc, dim, N, res):
+    """lowFilter is data * exp(-floor(r/att*100)/100) with r the integer
+    squared distance to size/2, att = res^2/2, and zero past idx 2000
+    (res 5: att 12.5 is where the float quotient moves idx across integers)."""
+    rng = np.random.default_rng(3)
+    C = 2
+    cells = N ** dim
+    d = (rng.standard_normal((C, cells)) + 1j * rng.standard_normal((C, cells))).astype(np.complex64)
+    out = orc.lowfilter(dim, N, d, res)
+    ax = np.arange(N) - N // 2
+    if dim == 3:
+        z, y, x = np.meshgrid(ax, ax, ax, indexing="ij")
+        r = (x * x + y * y + z * z).ravel()
+    else:
+        y, x = np.meshgrid(ax, ax, indexing="ij")
+        r = (x * x + y * y).ravel()
+    # the quotient is a float (int / float in the reference), the scale double
+    q = r.astype(np.float32) / np.float32(res * res / 2.0)
+    idx = np.floor(q.astype(np.float64) * 100.0).astype(np.int64)
+    w = np.where(idx >= 2000, 0.0, np.exp(-idx / 100.0))
+    ref = d.astype(np.complex128) * w
+    # expf of a float argument: |arg| <= 20 carries up to 20 * 2^-24 relative
+    np.testing.assert_allclose(out, ref, rtol=2e-6, atol=0)
+    centre = np.flatnonzero(r == 0)
+    assert np.array_equal(out[:, centre], d[:, centre])
+    assert np.all(out[:, idx >= 2000] == 0)
+
+
+@pytest.mark.parametrize("dim,N,C", [(2, 32, 4), (3, 16, 3)])
+def test_normalize_unit_coil_norm(orc, dim, N, C):
+    rng = np.random.default_rng(4)
+    cells = N ** dim
+    d = (rng.standard_normal((C, cells)) + 1j * rng.standard_normal((C, cells))).astype(np.complex64)
+    out = orc.normalize(dim, N, d)
+    norm = np.sum(np.abs(out.astype(np.complex128)) ** 2, axis=0)
+    np.testing.assert_allclose(norm, 1.0, rtol=1e-6)
+    # the per-voxel scale is real and common to all coils
+    ratio = out.astype(np.complex128) / d.astype(np.complex128)
+    np.testing.assert_allclose(ratio.imag, 0, atol=1e-6)
+    np.testing.assert_allclose(ratio, np.broadcast_to(ratio[:1], ratio.shape), rtol=1e-6)
+    np.testing.assert_allclose(ratio[0].real, 1 / np.sqrt(np.sum(np.abs(d.astype(np.complex128)) ** 2, 0)),
+                               rtol=1e-6)
+
+
+@pytest.mark.parametrize("dim,N,rc", [(2, 32, (16, 16, 1)), (3, 16, (8, 8, 8))])
+def test_sos_map_closed_forms(orc, dim, N, rc):
+    """SOS(map) with map == data is the SOS squared (real); scaling the map
+    by a complex a scales the result by conj(a); the crop is centred."""
+    rng = np.random.default_rng(5)
+    C = 3
+    cells = N ** dim
+    d = (rng.standard_normal((C, cells)) + 1j * rng.standard_normal((C, cells))).astype(np.complex64)
+    m = (rng.standard_normal((C, cells)) + 1j * rng.standard_normal((C, cells))).astype(np.complex64)
+    same = orc.sos_map(dim, N, d, d, rc)
+    s = orc.sos(dim, N, d, rc)
+    np.testing.assert_allclose(same.real, s.astype(np.float64) ** 2, rtol=1e-6)
+    assert np.all(same.imag == 0)
+    a = np.complex64(0.5 - 2.0j)
+    r1 = orc.sos_map(dim, N, d, m, rc).astype(np.complex128)
+    r2 = orc.sos_map(dim, N, d, (m * a).astype(np.complex64), rc).astype(np.complex128)
+    np.testing.assert_allclose(r2, np.conj(a) * r1, rtol=1e-5, atol=1e-5)
+    # brute force in complex128 over the centred crop
+    shape = (N,) * dim
+    dd = d.reshape((C,) + shape).astype(np.complex128)
+    mm = m.reshape((C,) + shape).astype(np.complex128)
+    rcx, rcy, rcz = rc
+    x0, y0 = (N - rcx) // 2, (N - rcy) // 2
+    if dim == 3:
+        z0 = (N - rcz) // 2
+        sl = (slice(None), slice(z0, z0 + rcz), slice(y0, y0 + rcy), slice(x0, x0 + rcx))
+    else:
+        sl = (slice(None), slice(y0, y0 + rcy), slice(x0, x0 + rcx))
+    ref = np.sum(dd[sl] * np.conj(mm[sl]), axis=0).ravel()
+    np.testing.assert_allclose(r1, ref, rtol=1e-5, atol=1e-5)
