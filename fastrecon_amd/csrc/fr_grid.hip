@@ -237,6 +237,7 @@ __global__ void __launch_bounds__(64) k_grid_items(const fr_item *__restrict__ i
     }
 }
 
+
 // Heavy units: sum their slices' partial boxes in slice order, write the grid.
 __global__ void k_combine(const fr_heavy *__restrict__ heavy, const fr_unit *__restrict__ units,
                           const float2 *__restrict__ partial, int64_t nslots, int cells_max, int C, int N, size_t cells,
@@ -289,13 +290,13 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
             p->partial_bytes = need;
         }
     }
-    auto kern = k_grid_items<ACC, TS>;
-    FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (p->nitems == 0) return FR_OK;
     const size_t cells = fr_cells(p->dim, p->N);
     const int64_t light = p->nitems - p->nitems_front;
     const int64_t span = 8 * XCD_RUN;
     dim3 grid((unsigned)(p->nitems_front + span * ((light + span - 1) / span)), chunks);
+    auto kern = k_grid_items<ACC == FR_ACC_LDS_ATOMIC ? 1 : 0, TS>;
+    FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, grid, dim3(64), lds, ctx->stream, p->items, p->nitems, p->units, p->hdrs, p->taps,
                        (const TS *)s->data, s->dcf, s->C, stride, acc_floats, (int)p->N, cells, out,
                        (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
@@ -312,9 +313,13 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
 
 fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int)
 {
-    if (s->type == FR_SAMPLES_C64)
-        return p->accumulate ? launch<1, float2>(p, s, out) : launch<0, float2>(p, s, out);
-    return p->accumulate ? launch<1, __half2>(p, s, out) : launch<0, __half2>(p, s, out);
+    const bool c64 = s->type == FR_SAMPLES_C64;
+    switch (p->accumulate) {
+    case FR_ACC_LDS_ATOMIC:
+        return c64 ? launch<FR_ACC_LDS_ATOMIC, float2>(p, s, out) : launch<FR_ACC_LDS_ATOMIC, __half2>(p, s, out);
+    default:
+        return c64 ? launch<FR_ACC_LDS_RMW, float2>(p, s, out) : launch<FR_ACC_LDS_RMW, __half2>(p, s, out);
+    }
 }
 
 extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *out)

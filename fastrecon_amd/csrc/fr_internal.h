@@ -49,6 +49,10 @@ struct fr_grids {
 // (row cells_max) absorbs the padding taps that fill records up to a
 // multiple of FR_TAP_GROUP.
 #define FR_TAP_GROUP 4
+// accumulation modes of a plan (fr_tune.accumulate after resolution)
+#define FR_ACC_LDS_ATOMIC 1
+#define FR_ACC_LDS_RMW 2
+#define FR_ACC_REGS 3       // refused: see fr_plan_create
 
 // A work unit is a box of grid cells owned by one wave.  Its records (one
 // per sample that has taps in the box, ascending sample order) are H[hdr ..

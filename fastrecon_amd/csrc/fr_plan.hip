@@ -162,7 +162,7 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
                                const int32_t *__restrict__ cell_unit, const fr_unit *__restrict__ units,
                                const uint32_t *__restrict__ keys, const uint64_t *__restrict__ vals,
                                const uint64_t *__restrict__ toff, int64_t R, const float *__restrict__ lut,
-                               uint32_t dummy_row, uint2 *__restrict__ H, uint2 *__restrict__ T)
+                               uint32_t tap_scale, uint32_t pad_x, uint2 *__restrict__ H, uint2 *__restrict__ T)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
@@ -180,10 +180,10 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
         const int ux = U.ext & 255, uy = (U.ext >> 8) & 255;
         fr_box_taps(b, lo, hi, g.hw, g.L, [&](int x, int y, int z, int ki) {
             const uint32_t cell = (uint32_t)(((z - U.oz) * uy + (y - U.oy)) * ux + (x - U.ox));
-            w[i++] = make_uint2(cell * (4u * FR_ACC_STRIDE), __float_as_uint(lut[ki]));
+            w[i++] = make_uint2(cell * tap_scale, __float_as_uint(lut[ki]));
         });
     });
-    for (; i < np; ++i) w[i] = make_uint2(dummy_row * (4u * FR_ACC_STRIDE), 0u);
+    for (; i < np; ++i) w[i] = make_uint2(pad_x, 0u);
 }
 
 // A record opens a new work item when it is its unit's first record or when
@@ -361,6 +361,13 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     (void)stride;
     p->geom = geom;
     p->accumulate = tune ? tune->accumulate : 0;
+    FR_CHECK(ctx, p->accumulate >= 0 && p->accumulate <= FR_ACC_REGS, FR_ERR_INVALID, "unknown accumulate mode %d",
+             p->accumulate);
+    // VGPR accumulation (GPR-index addressed adds) gave wrong sums and a
+    // memory fault on gfx950 whose cause is not established: it is refused.
+    FR_CHECK(ctx, p->accumulate != FR_ACC_REGS, FR_ERR_UNSUPPORTED,
+             "register accumulation (accumulate = 3) is disabled in this build");
+    if (p->accumulate == 0) p->accumulate = FR_ACC_LDS_RMW;
     p->exact_order = tune ? (tune->exact_order != 0) : 0;
 
     const int N = (int)p->N;
@@ -500,13 +507,16 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     if ((e = fr_alloc(ctx, (void **)&p->taps, sizeof(uint2) * (TT + 576)))) return e;
     FR_HIP(ctx, hipMemsetAsync(p->taps, 0, sizeof(uint2) * (TT + 576), st));
     FR_HIP(ctx, hipMemsetAsync(p->hdrs, 0, sizeof(uint2) * (R + 64), st));
+    // tap cell field: the byte offset of the cell's accumulator row; padding
+    // taps go to the spare row cells_max with weight 0
+    const uint32_t tap_scale = 4u * FR_ACC_STRIDE;
+    const uint32_t pad_x = (uint32_t)geom.cells_max * 4u * FR_ACC_STRIDE;
     if (R) {
         hipLaunchKernelGGL(k_unit_ranges, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, (int64_t)R,
                            p->units);
         hipLaunchKernelGGL(k_unit_counts, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, toff.p);
         hipLaunchKernelGGL(k_fill_records, dim3(nblk(R, 128)), dim3(128), 0, st, s->traj, M, g, cu_d.p, p->units,
-                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, (uint32_t)geom.cells_max, p->hdrs,
-                           p->taps);
+                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, tap_scale, pad_x, p->hdrs, p->taps);
     }
     FR_HIP(ctx, hipGetLastError());
 

@@ -56,7 +56,10 @@ typedef struct fr_fft fr_fft;
 typedef struct {
     int tile_x, tile_y, tile_z;  /* base work-unit box (one wave owns a unit) */
     int slabs;                   /* waves per workgroup                        */
-    int accumulate;              /* 0: LDS read-add-write, 1: LDS float atomic add */
+    int accumulate;              /* where a unit's cells accumulate:
+                                    0: automatic (LDS read-add-write),
+                                    1: LDS float atomic add, 2: LDS read-add-write,
+                                    3: VGPRs -- refused (FR_ERR_UNSUPPORTED) */
     int split_taps;              /* heavier units are split (0: automatic)      */
     int exact_order;             /* 1: split units by space only, so every cell
                                     sums in ascending sample order (bit-identical

@@ -83,15 +83,29 @@ def test_kooshball3d_small_bitwise(ctx, orc, coils):
     check_exact(got, ref, f"koosh C={coils}")
 
 
-@pytest.mark.parametrize("tune", [dict(accumulate=1), dict(tile_x=8, tile_y=4, tile_z=2),
-                                  dict(tile_x=4, tile_y=4, tile_z=4), dict(split_taps=64),
-                                  dict(split_taps=1)])
+REGS_OFF = pytest.mark.skip(reason="VGPR accumulation (accumulate=3) gave wrong sums and a GPU memory fault on "
+                                   "gfx950 with no established cause; the mode is refused by the plan")
+
+
+@pytest.mark.parametrize("tune", [dict(accumulate=1), dict(accumulate=2),
+                                  pytest.param(dict(accumulate=3), marks=REGS_OFF),
+                                  dict(tile_x=8, tile_y=4, tile_z=2), dict(tile_x=4, tile_y=4, tile_z=4),
+                                  pytest.param(dict(tile_x=2, tile_y=4, tile_z=4, accumulate=3), marks=REGS_OFF),
+                                  dict(tile_x=8, tile_y=2, tile_z=2), dict(tile_x=3, tile_y=3, tile_z=3),
+                                  dict(split_taps=64), dict(split_taps=1), dict(split_taps=1, accumulate=2)])
 def test_tile_shape_and_accumulate_mode_invariance(ctx, orc, tune):
     tn, dcf, data, N, lut = case(orc, "kooshball3d", 300, 48, 6, seed=5, image=24)
     got, info = run_grid(ctx, tn, dcf, data, N, lut, tune=dict(tune, exact_order=1))
     assert info["tile_x"] == tune.get("tile_x", info["tile_x"])
     ref = orc.grid_f32(tn, dcf, data, N, 4.0, lut, nthreads=8)
     check_exact(got, ref, f"tune {tune}")
+
+
+def test_register_mode_is_refused(ctx, orc):
+    """The plan refuses VGPR accumulation (accumulate=3) before any launch."""
+    tn, dcf, data, N, lut = case(orc, "kooshball3d", 50, 32, 2, seed=5, image=16)
+    with pytest.raises(hip.HipError, match="disabled"):
+        run_grid(ctx, tn, dcf, data, N, lut, tune=dict(accumulate=3))
 
 
 def test_edges_integer_centres_and_outside_samples(ctx, orc):
@@ -151,6 +165,7 @@ def test_update_acquisition(ctx, orc):
 
 
 @pytest.mark.parametrize("coils,tune", [(32, {}), (8, {}), (40, {}), (32, dict(split_taps=300)),
+                                        (32, dict(split_taps=300, accumulate=2)),
                                         (32, dict(split_taps=2000, tile_x=4, tile_y=4, tile_z=4))])
 def test_sliced_mode_parity_and_determinism(ctx, orc, coils, tune):
     """Default mode: heavy units are cut by sample range; the slices are
