@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recon", action="store_true")
+    ap.add_argument("--no-ldcf", action="store_true", help="skip the learned-vs-iterative DCF leg")
     ap.add_argument("--accumulate", type=int, default=0)
     ap.add_argument("--exact", action="store_true", help="exact-order plan for the timed steps")
     ap.add_argument("--unit", type=int, nargs=3, default=None, metavar=("X", "Y", "Z"),
@@ -236,6 +237,28 @@ def main():
         out["recon_stages_ms"] = {"grid": float(r[0]), "fft": float(r[1]), "shift_crop_sos": float(r[2])}
         fft.close()
         img.close()
+
+    if not a.no_ldcf:
+        # learned vs iterative DCF on this workload's geometry (the network's
+        # checkpoint was trained on the device by fastrecon_amd.ldcf_train)
+        from fastrecon_amd import ldcf
+        grids.close()
+        plan.close()
+        s.close()
+        torch.cuda.empty_cache()
+        mlp, meta = ldcf.load(ctx)
+
+        def fill(smp):
+            gen2 = torch.Generator(device="cuda")
+            gen2.manual_seed(2)
+            d = torch.randn(M * C * 2, generator=gen2, device="cuda", dtype=torch.float32)
+            torch.cuda.synchronize()
+            smp.set_data_device(d.data_ptr(), hip.FR_LAYOUT_SAMPLE_MAJOR)
+            ctx.sync()
+
+        r = ldcf.evaluate(ctx, mlp, traj, fill, N, (a.image, a.image, a.image), ramp=dcf, reps=2, coils=C)
+        out["ldcf"] = dict(r, iterations=10, checkpoint_rows=meta.get("rows"), checkpoint_loss=meta["losses"][-1][1])
+        mlp.close()
 
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(traj_n, dcf, N, C, lut, a.cpu_seconds)

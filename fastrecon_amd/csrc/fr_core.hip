@@ -102,6 +102,68 @@ extern "C" fr_status fr_samples_create(fr_ctx *ctx, int dim, int64_t M, int coil
     return FR_OK;
 }
 
+__global__ void k_frame_rows(float *__restrict__ row, const int64_t *__restrict__ off, int frames, int64_t M)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    int lo = 0, hi = frames - 1;               // last frame whose offset <= i
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    row[i] = (float)lo;
+}
+
+extern "C" fr_status fr_samples_create_frames(fr_ctx *ctx, int frames, const int64_t *frame_samples, int coils,
+                                              int type, fr_samples **out)
+{
+    if (!ctx || !out || !frame_samples) return FR_ERR_INVALID;
+    *out = nullptr;
+    FR_CHECK(ctx, frames >= 1 && frames <= (1 << 16), FR_ERR_INVALID, "frame count %d outside [1, 65536]", frames);
+    std::vector<int64_t> off(frames + 1, 0);
+    for (int f = 0; f < frames; ++f) {
+        FR_CHECK(ctx, frame_samples[f] >= 1, FR_ERR_INVALID, "frame %d has no samples", f);
+        off[f + 1] = off[f] + frame_samples[f];
+    }
+    const int64_t M = off[frames];
+    FR_CHECK(ctx, M < (int64_t(1) << 27), FR_ERR_UNSUPPORTED, "sample count %lld outside [1, 2^27)", (long long)M);
+    FR_CHECK(ctx, coils > 0 && coils <= 1024, FR_ERR_INVALID, "coil count %d outside [1,1024]", coils);
+    FR_CHECK(ctx, type == FR_SAMPLES_C64 || type == FR_SAMPLES_C16, FR_ERR_INVALID, "bad sample type");
+    fr_samples *s = new (std::nothrow) fr_samples;
+    if (!s) return FR_ERR_NOMEM;
+    s->ctx = ctx; s->dim = 2; s->M = M; s->C = coils; s->type = type;
+    s->frames = frames;
+    s->frame_off = off;
+    s->coil_set.assign(coils, 0);
+    const size_t esz = type == FR_SAMPLES_C64 ? 8 : 4;
+    int64_t *off_d = nullptr;
+    fr_status st;
+    if ((st = fr_alloc(ctx, (void **)&s->traj, sizeof(float) * 3 * M)) ||
+        (st = fr_alloc(ctx, (void **)&s->dcf, sizeof(float) * M)) ||
+        (st = fr_alloc(ctx, &s->data, esz * M * coils)) ||
+        (st = fr_alloc(ctx, &s->staging, 8 * M + 16 * M)) ||
+        (st = fr_alloc(ctx, (void **)&off_d, sizeof(int64_t) * (frames + 1)))) {
+        hipFree(off_d);
+        fr_samples_destroy(s);
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(off_d, off.data(), sizeof(int64_t) * (frames + 1), hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_frame_rows, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ctx->stream,
+                           s->traj + 2 * M, off_d, frames, M);
+        e = hipMemsetAsync(s->data, 0, esz * M * coils, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(off_d);
+    if (e != hipSuccess) {
+        fr_samples_destroy(s);
+        return fr_fail(ctx, FR_ERR_HIP, "frame samples: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return FR_OK;
+}
+
 extern "C" fr_status fr_samples_destroy(fr_samples *s)
 {
     if (!s) return FR_OK;
@@ -319,6 +381,22 @@ extern "C" fr_status fr_grids_create(fr_ctx *ctx, int dim, unsigned N, int coils
     fr_grids *g = new (std::nothrow) fr_grids;
     if (!g) return FR_ERR_NOMEM;
     g->ctx = ctx; g->dim = dim; g->N = N; g->C = coils; g->cells = fr_cells(dim, N);
+    fr_status st = fr_alloc(ctx, (void **)&g->data, sizeof(float2) * g->cells * coils);
+    if (st) { delete g; return st; }
+    *out = g;
+    return FR_OK;
+}
+
+extern "C" fr_status fr_grids_create_frames(fr_ctx *ctx, unsigned N, int coils, int frames, fr_grids **out)
+{
+    if (!ctx || !out) return FR_ERR_INVALID;
+    *out = nullptr;
+    FR_CHECK(ctx, N >= 2 && N <= 4096, FR_ERR_INVALID, "grid size %u outside [2,4096]", N);
+    FR_CHECK(ctx, coils >= 1 && frames >= 1, FR_ERR_INVALID, "coils and frames must be >= 1");
+    fr_grids *g = new (std::nothrow) fr_grids;
+    if (!g) return FR_ERR_NOMEM;
+    g->ctx = ctx; g->dim = 2; g->N = N; g->C = coils; g->frames = frames;
+    g->cells = (size_t)frames * N * N;
     fr_status st = fr_alloc(ctx, (void **)&g->data, sizeof(float2) * g->cells * coils);
     if (st) { delete g; return st; }
     *out = g;

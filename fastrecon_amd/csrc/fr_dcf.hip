@@ -6,6 +6,7 @@
 #include "fr_taps.h"
 
 #include <new>
+#include <vector>
 
 namespace {
 
@@ -13,14 +14,14 @@ namespace {
 // LUT weight * grid[c][cell], in the reference's z, y, x tap order.
 __global__ void k_degrid(const float *__restrict__ traj, int64_t M, int dim, int N, float hw, int L,
                          const float *__restrict__ lut, const float2 *__restrict__ grid, size_t cells, int C,
-                         float2 *__restrict__ out)
+                         float2 *__restrict__ out, int frames)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M * C) return;
     const int64_t s = i / C;
     const int c = (int)(i - s * C);
     fr_box b;
-    fr_sample_box(dim, N, traj, M, s, hw, b);
+    fr_sample_box(dim, N, traj, M, s, hw, b, frames);
     const float2 *g = grid + (size_t)c * cells;
     float re = 0.f, im = 0.f;
     fr_box_taps(b, b.s, b.e, hw, L, [&](int x, int y, int z, int ki) {
@@ -61,7 +62,8 @@ fr_status fr_degrid_raw(fr_plan *p, const fr_samples *s, const float2 *grid, int
 {
     fr_ctx *ctx = p->ctx;
     hipLaunchKernelGGL(k_degrid, dim3(nblk(p->M * C, 256)), dim3(256), 0, ctx->stream, s->traj, p->M, p->dim,
-                       (int)p->N, p->kHW, p->L, p->lut, grid, fr_cells(p->dim, p->N), C, out);
+                       (int)p->N, p->kHW, p->L, p->lut, grid, fr_plan_cells(p->dim, p->N, p->frames), C, out,
+                       p->frames);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
@@ -71,7 +73,8 @@ extern "C" fr_status fr_degrid_execute(fr_plan *p, const fr_samples *s, const fr
     if (!p || !s || !g || !out_dev) return FR_ERR_INVALID;
     fr_ctx *ctx = p->ctx;
     FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID, "samples do not match the plan");
-    FR_CHECK(ctx, g->dim == p->dim && g->N == p->N, FR_ERR_INVALID, "grids do not match the plan");
+    FR_CHECK(ctx, g->dim == p->dim && g->N == p->N && g->frames == p->frames && s->frames == p->frames,
+             FR_ERR_INVALID, "grids do not match the plan");
     return fr_degrid_raw(p, s, g->data, g->C, (float2 *)out_dev);
 }
 
@@ -86,11 +89,22 @@ extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int
     fr_samples *u = nullptr;
     fr_grids *g = nullptr;
     float2 *den = nullptr;
-    fr_status st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &u);
-    if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &g);
+    FR_CHECK(ctx, s->frames == p->frames, FR_ERR_INVALID, "samples and plan differ in frames");
+    fr_status st;
+    int rows = s->dim;
+    if (s->frames > 0) {
+        std::vector<int64_t> cnt(s->frames);
+        for (int f = 0; f < s->frames; ++f) cnt[f] = s->frame_off[f + 1] - s->frame_off[f];
+        st = fr_samples_create_frames(ctx, s->frames, cnt.data(), 1, FR_SAMPLES_C64, &u);
+        if (!st) st = fr_grids_create_frames(ctx, p->N, 1, s->frames, &g);
+        rows = 3;
+    } else {
+        st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &u);
+        if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &g);
+    }
     if (!st) st = fr_alloc(ctx, (void **)&den, sizeof(float2) * M);
     if (!st) {
-        hipMemcpyAsync(u->traj, s->traj, sizeof(float) * s->dim * M, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(u->traj, s->traj, sizeof(float) * rows * M, hipMemcpyDeviceToDevice, ctx->stream);
         hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, (float2 *)u->data, M);
         if (init_ones || !s->has_dcf)
             hipLaunchKernelGGL(k_fill_f32, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, M, 1.f);

@@ -291,7 +291,7 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
         }
     }
     if (p->nitems == 0) return FR_OK;
-    const size_t cells = fr_cells(p->dim, p->N);
+    const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int64_t light = p->nitems - p->nitems_front;
     const int64_t span = 8 * XCD_RUN;
     dim3 grid((unsigned)(p->nitems_front + span * ((light + span - 1) / span)), chunks);
@@ -329,9 +329,11 @@ extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *
     FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID,
              "samples (dim %d, M %lld) do not match the plan (dim %d, M %lld)", s->dim, (long long)s->M, p->dim,
              (long long)p->M);
-    FR_CHECK(ctx, out->dim == p->dim && out->N == p->N && out->C == s->C, FR_ERR_INVALID,
-             "grids (dim %d, N %u, C %d) do not match plan/samples (dim %d, N %u, C %d)", out->dim, out->N, out->C,
-             p->dim, p->N, s->C);
+    FR_CHECK(ctx, out->dim == p->dim && out->N == p->N && out->C == s->C && out->frames == p->frames,
+             FR_ERR_INVALID,
+             "grids (dim %d, N %u, C %d, frames %d) do not match plan/samples (dim %d, N %u, C %d, frames %d)",
+             out->dim, out->N, out->C, out->frames, p->dim, p->N, s->C, p->frames);
+    FR_CHECK(ctx, s->frames == p->frames, FR_ERR_INVALID, "samples and plan differ in frames");
     FR_CHECK(ctx, s->has_dcf, FR_ERR_STATE, "DCF not set");
     FR_CHECK(ctx, p->geom.cells_max <= 64, FR_ERR_INVALID, "plan unit box exceeds 64 cells");
     return fr_launch_grid(p, s, out->data, 0);

@@ -2,6 +2,8 @@
 // fused shift + crop + sum-of-squares (+ deapodisation) epilogue.
 #include "fr_internal.h"
 
+#include <algorithm>
+
 #include <rocfft/rocfft.h>
 
 #include <mutex>
@@ -42,14 +44,16 @@ __global__ void k_fftshift(float2 *__restrict__ g, int dim, int N, int C, size_t
 // 173-176); source index optionally fftshifted; sum over coils in order of
 // fl(re*re) + fl(im*im) (the real part of v * conj(v)), then sqrt
 // (ImageFilter.cpp:162-167) and |.| (Main.cpp:177), divided by deapod.
+// Batched frames (frames > 0): z is the frame, taken whole and unshifted,
+// and the deapodisation map is one rcx x rcy image shared by the frames.
 __global__ void k_sos(const float2 *__restrict__ g, int dim, int N, int C, size_t cells, int rcx, int rcy, int rcz,
-                      int shift, const float *__restrict__ deapod, float *__restrict__ out)
+                      int shift, const float *__restrict__ deapod, float *__restrict__ out, int frames)
 {
     const size_t n = (size_t)rcx * rcy * rcz;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int x = (int)(i % rcx), y = (int)((i / rcx) % rcy), z = (int)(i / ((size_t)rcx * rcy));
-    const int nz = dim == 3 ? N : 1;
+    const int nz = frames > 0 ? frames : (dim == 3 ? N : 1);
     int sx = (N - rcx) / 2 + x, sy = (N - rcy) / 2 + y, sz = (nz - rcz) / 2 + z;
     if (shift) {
         const int h = N / 2;
@@ -64,7 +68,7 @@ __global__ void k_sos(const float2 *__restrict__ g, int dim, int N, int C, size_
         acc = __fadd_rn(acc, __fadd_rn(__fmul_rn(v.x, v.x), __fmul_rn(v.y, v.y)));
     }
     float m = sqrtf(acc);
-    if (deapod) m = __fdiv_rn(m, deapod[i]);
+    if (deapod) m = __fdiv_rn(m, deapod[frames > 0 ? i % ((size_t)rcx * rcy) : i]);
     out[i] = m;
 }
 
@@ -121,7 +125,8 @@ extern "C" fr_status fr_fft_execute(fr_fft *f, fr_grids *g)
 {
     if (!f || !g) return FR_ERR_INVALID;
     fr_ctx *ctx = f->ctx;
-    FR_CHECK(ctx, g->dim == f->dim && g->N == f->N && g->C == f->batch, FR_ERR_INVALID,
+    FR_CHECK(ctx, g->dim == f->dim && g->N == f->N && (int64_t)g->C * std::max(g->frames, 1) == f->batch,
+             FR_ERR_INVALID,
              "grids (dim %d, N %u, C %d) do not match the FFT plan (dim %d, N %u, batch %d)", g->dim, g->N, g->C,
              f->dim, f->N, f->batch);
     void *bufs[1] = {g->data};
@@ -134,9 +139,12 @@ extern "C" fr_status fr_fftshift(fr_ctx *ctx, fr_grids *g)
 {
     if (!ctx || !g) return FR_ERR_INVALID;
     FR_CHECK(ctx, g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size (got %u)", g->N);
-    const size_t n = g->cells / 2 * g->C;
-    hipLaunchKernelGGL(k_fftshift, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N, g->C,
-                       g->cells);
+    // batched frames: every (coil, frame) is its own contiguous 2D grid
+    const int batch = g->C * std::max(g->frames, 1);
+    const size_t cells = g->frames > 0 ? (size_t)g->N * g->N : g->cells;
+    const size_t n = cells / 2 * batch;
+    hipLaunchKernelGGL(k_fftshift, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N, batch,
+                       cells);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
@@ -145,13 +153,13 @@ extern "C" fr_status fr_sos(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsign
                             const float *deapod_dev, float *out_dev)
 {
     if (!ctx || !g || !out_dev) return FR_ERR_INVALID;
-    if (g->dim == 2) rcz = 1;
+    if (g->dim == 2) rcz = g->frames > 0 ? (unsigned)g->frames : 1u;    // frames: one image per frame
     FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
              FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
     FR_CHECK(ctx, !apply_shift || g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size");
     const size_t n = (size_t)rcx * rcy * rcz;
     hipLaunchKernelGGL(k_sos, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N, g->C,
-                       g->cells, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev);
+                       g->cells, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev, g->frames);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
@@ -285,6 +293,7 @@ __global__ void k_sos_map(const float2 *__restrict__ g, const float2 *__restrict
 extern "C" fr_status fr_lowfilter(fr_ctx *ctx, fr_grids *g, int res)
 {
     if (!ctx || !g) return FR_ERR_INVALID;
+    FR_CHECK(ctx, g->frames == 0, FR_ERR_UNSUPPORTED, "not available for batched frames");
     FR_CHECK(ctx, res > 0, FR_ERR_INVALID, "lowFilter resolution must be positive");
     // coefficients exactly as the reference's host loop: expf(-r / 100.0)
     float coeff[2000];
@@ -308,6 +317,7 @@ extern "C" fr_status fr_lowfilter(fr_ctx *ctx, fr_grids *g, int res)
 extern "C" fr_status fr_normalize(fr_ctx *ctx, fr_grids *g)
 {
     if (!ctx || !g) return FR_ERR_INVALID;
+    FR_CHECK(ctx, g->frames == 0, FR_ERR_UNSUPPORTED, "not available for batched frames");
     hipLaunchKernelGGL(k_normalize, dim3(nblk(g->cells, 256)), dim3(256), 0, ctx->stream, g->data, g->C, g->cells);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
@@ -316,6 +326,7 @@ extern "C" fr_status fr_normalize(fr_ctx *ctx, fr_grids *g)
 extern "C" fr_status fr_crop(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz, void *out_dev)
 {
     if (!ctx || !g || !out_dev) return FR_ERR_INVALID;
+    FR_CHECK(ctx, g->frames == 0, FR_ERR_UNSUPPORTED, "not available for batched frames");
     if (g->dim == 2) rcz = 1;
     FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
              FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
@@ -330,6 +341,7 @@ extern "C" fr_status fr_sos_map(fr_ctx *ctx, const fr_grids *g, const fr_grids *
                                 unsigned rcz, int apply_shift, void *out_dev)
 {
     if (!ctx || !g || !map || !out_dev) return FR_ERR_INVALID;
+    FR_CHECK(ctx, g->frames == 0, FR_ERR_UNSUPPORTED, "not available for batched frames");
     if (g->dim == 2) rcz = 1;
     FR_CHECK(ctx, map->dim == g->dim && map->N == g->N && map->C >= g->C, FR_ERR_INVALID,
              "map grids do not match the image grids");

@@ -31,6 +31,10 @@ struct fr_samples {
     void *staging = nullptr;   // M complex64 scratch for host uploads
     bool has_traj = false, has_dcf = false;
     std::vector<char> coil_set;
+    // batched frames: samples [frame_off[f], frame_off[f+1]) belong to 2D
+    // frame f, and traj holds a third row with the frame index (as float)
+    int frames = 0;
+    std::vector<int64_t> frame_off;
 };
 
 struct fr_grids {
@@ -38,8 +42,9 @@ struct fr_grids {
     int dim = 0;
     unsigned N = 0;
     int C = 0;
-    size_t cells = 0;          // N^dim
-    float2 *data = nullptr;    // [C][cells]
+    size_t cells = 0;          // N^dim, or frames * N^2 for batched frames
+    float2 *data = nullptr;    // [C][cells] (frames: [C][frame][N^2])
+    int frames = 0;            // batched 2D frames (0: one grid)
 };
 
 // Accumulator row of one grid cell in LDS: 64 components (32 coils, re/im)
@@ -112,6 +117,7 @@ struct fr_plan {
     int64_t nrecords = 0, ntaps = 0;
     int64_t max_unit_taps = 0;
     int exact_order = 0;               // 1: no sample slicing (bitwise CPU order)
+    int frames = 0;                    // batched 2D frames (0: one grid)
     fr_item *items = nullptr;          // device, launch order
     int64_t nitems = 0;
     int64_t nitems_front = 0;          // items launched first, one per workgroup
@@ -155,6 +161,12 @@ fr_status fr_alloc(fr_ctx *ctx, void **p, size_t bytes);
 static inline size_t fr_cells(int dim, unsigned N)
 {
     return dim == 3 ? (size_t)N * N * N : (size_t)N * N;
+}
+
+// cells of one coil of a plan's grid (all frames of a batched plan)
+static inline size_t fr_plan_cells(int dim, unsigned N, int frames)
+{
+    return frames > 0 ? (size_t)frames * N * N : fr_cells(dim, N);
 }
 
 // Launch helpers implemented in the kernel translation units.

@@ -1,0 +1,604 @@
+// Learned density compensation (net-new; the reference has no DCF
+// estimation): per-sample trajectory features and a 6-64-64-1 SiLU MLP with
+// BF16 weight matrices and FP32 accumulation, trained and applied on the
+// device.
+//
+// Features (one row of 8 floats per sample, the last two zero):
+//   u = (t - (N-1)/2) / (N/2) for the normalised point t,
+//   f0 = |u|, f1 = atan2(u_y, u_x)/pi, f2 = acos(u_z/|u|)/pi (0.5 in 2D),
+//   f3..f5 = log1p(n_r)/8 with n_r the number of other samples whose nearest
+//   cell lies within Chebyshev distance r = 1, 2, 4 of this sample's cell.
+// The counts come from a cell histogram and its summed-area table, so each
+// sample costs 3 box sums (8 corner reads each in 3D) whatever the density.
+//
+// MLP layer 2 (64 x 64) runs on MFMA (v_mfma_f32_16x16x32_bf16): a wave
+// takes 16 samples, computes h1 = silu(x W1 + b1) for them in VGPRs already
+// in the B-operand layout (lane l: sample l%16, hidden units 32ks+8(l/16)+j),
+// and multiplies by W2^T held as eight A fragments; the D fragments (lane l:
+// sample l%16, outputs 16mt+4(l/16)+i) feed SiLU and the 64 -> 1 layer, whose
+// four partial sums per sample are added across lane groups in a fixed order.
+// Training recomputes this forward, back-propagates in FP32 and accumulates
+// each wave's weight gradient in registers; per-wave partials are summed in
+// wave order by a second kernel (deterministic), then Adam updates the FP32
+// master weights and refreshes the BF16 copies.
+#include "fr_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+constexpr int NF = 6;
+constexpr int H = 64;
+constexpr int FSTRIDE = 8;   // floats per feature row
+constexpr int O_W1 = 0, O_B1 = NF * H, O_W2 = O_B1 + H, O_B2 = O_W2 + H * H, O_W3 = O_B2 + H, O_B3 = O_W3 + H;
+constexpr int NPARAM = O_B3 + 1;
+constexpr int SLOT = NPARAM + 1;   // per-wave partial: gradient + loss
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__host__ __device__ inline uint32_t f2bf(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;   // round to nearest even (finite values)
+}
+
+__device__ inline float bf2f(uint32_t b) { return __uint_as_float(b << 16); }
+
+__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+__device__ inline float dsilu(float z)
+{
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// ------------------------------------------------------------- features
+__global__ void k_cell_hist(const float *__restrict__ traj, int64_t M, int dim, int N, uint32_t *__restrict__ hist)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    size_t cell = 0;
+    for (int d = dim - 1; d >= 0; --d) {
+        int c = (int)floorf(traj[d * M + s] + 0.5f);
+        c = min(max(c, 0), N - 1);
+        cell = cell * N + c;
+    }
+    atomicAdd(&hist[cell], 1u);   // integer counts: the result does not depend on order
+}
+
+// inclusive prefix sum along one axis; axis stride `st`, lines enumerated by
+// the other two coordinates
+__global__ void k_prefix_axis(uint32_t *__restrict__ a, int N, int dim, int axis)
+{
+    const int64_t line = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lines = dim == 3 ? (int64_t)N * N : N;
+    if (line >= lines) return;
+    size_t base, st;
+    if (dim == 2) {
+        if (axis == 0) { base = (size_t)line * N; st = 1; }
+        else { base = (size_t)line; st = N; }
+    } else {
+        const int64_t p = line % N, q = line / N;
+        if (axis == 0) { base = ((size_t)q * N + p) * N; st = 1; }
+        else if (axis == 1) { base = (size_t)q * N * N + p; st = N; }
+        else { base = (size_t)q * N + p; st = (size_t)N * N; }
+    }
+    uint32_t run = 0;
+    for (int i = 0; i < N; ++i) {
+        run += a[base + i * st];
+        a[base + i * st] = run;
+    }
+}
+
+__device__ inline uint32_t sat_at(const uint32_t *__restrict__ S, int N, int x, int y, int z)
+{
+    if (x < 0 || y < 0 || z < 0) return 0u;
+    return S[((size_t)z * N + y) * N + x];
+}
+
+__device__ inline uint32_t box_count(const uint32_t *__restrict__ S, int dim, int N, const int *c, int r)
+{
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) {
+        lo[d] = max(c[d] - r, 0) - 1;
+        hi[d] = min(c[d] + r, N - 1);
+    }
+    if (dim == 2) {
+        return sat_at(S, N, hi[0], hi[1], 0) - sat_at(S, N, lo[0], hi[1], 0) - sat_at(S, N, hi[0], lo[1], 0) +
+               sat_at(S, N, lo[0], lo[1], 0);
+    }
+    return sat_at(S, N, hi[0], hi[1], hi[2]) - sat_at(S, N, lo[0], hi[1], hi[2]) - sat_at(S, N, hi[0], lo[1], hi[2]) -
+           sat_at(S, N, hi[0], hi[1], lo[2]) + sat_at(S, N, lo[0], lo[1], hi[2]) + sat_at(S, N, lo[0], hi[1], lo[2]) +
+           sat_at(S, N, hi[0], lo[1], lo[2]) - sat_at(S, N, lo[0], lo[1], lo[2]);
+}
+
+__global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, const uint32_t *__restrict__ S,
+                           float *__restrict__ feat)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const float cf = 0.5f * (float)(N - 1), hN = 0.5f * (float)N;
+    float u[3] = {0.f, 0.f, 0.f};
+    int c[3] = {0, 0, 0};
+    for (int d = 0; d < dim; ++d) {
+        const float t = traj[d * M + s];
+        u[d] = __fdiv_rn(__fsub_rn(t, cf), hN);
+        c[d] = min(max((int)floorf(t + 0.5f), 0), N - 1);
+    }
+    const float r2 = dim == 3 ? __fadd_rn(__fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1])), __fmul_rn(u[2], u[2]))
+                              : __fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1]));
+    const float r = sqrtf(r2);
+    const float inv_pi = 0.318309886183790671f;
+    float f[FSTRIDE];
+    f[0] = r;
+    f[1] = atan2f(u[1], u[0]) * inv_pi;
+    f[2] = (dim == 3 && r > 0.f) ? acosf(fminf(fmaxf(u[2] / r, -1.f), 1.f)) * inv_pi : 0.5f;
+    const int rr[3] = {1, 2, 4};
+    for (int q = 0; q < 3; ++q) f[3 + q] = log1pf((float)(box_count(S, dim, N, c, rr[q]) - 1u)) * 0.125f;
+    f[6] = 0.f;
+    f[7] = 0.f;
+    float4 *o = reinterpret_cast<float4 *>(feat + s * FSTRIDE);
+    o[0] = make_float4(f[0], f[1], f[2], f[3]);
+    o[1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
+// ------------------------------------------------------------------- MLP
+struct mlp_dev {
+    const float *p;          // FP32 master parameters (biases are read from here)
+    const uint16_t *w1;      // [6][64] BF16
+    const uint16_t *w2;      // [64 in][64 out] BF16
+    const uint16_t *w3;      // [64] BF16
+};
+
+// W2^T as A fragments: fragment (mt, ks), lane l holds W2[k][m] for
+// m = 16 mt + l%16, k = 32 ks + 8 (l/16) + j, j = 0..7.
+__device__ inline void load_a(const mlp_dev &W, int lane, u32x4 (&a)[4][2])
+{
+    const int m0 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            uint32_t w[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = W.w2[(32 * ks + 8 * g + j) * H + 16 * mt + m0];
+            a[mt][ks] = u32x4{w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16), w[6] | (w[7] << 16)};
+        }
+}
+
+// Forward for the 16 samples of one wave.  Lane l: sample l%16.
+// Outputs: z1[ks][j] (unit 32ks+8g+j), h1b (BF16-rounded h1 as float),
+// z2[mt][i] (output 16mt+4g+i), h2, y (identical in the 4 lanes of a sample).
+struct fwd_state {
+    float z1[2][8], h1[2][8];
+    float z2[4][4], h2[4][4];
+    float y;
+};
+
+__device__ inline void mlp_forward16(const mlp_dev &W, const u32x4 (&a)[4][2], const float *x, int lane, fwd_state &F)
+{
+    const int g = lane >> 4;
+    u32x4 b[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        uint32_t hb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 32 * ks + 8 * g + j;
+            float z = W.p[O_B1 + k];
+#pragma unroll
+            for (int i = 0; i < NF; ++i) z = __fadd_rn(z, __fmul_rn(x[i], bf2f(W.w1[i * H + k])));
+            F.z1[ks][j] = z;
+            hb[j] = f2bf(silu(z));
+            F.h1[ks][j] = bf2f(hb[j]);
+        }
+        b[ks] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16)};
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[mt][0]),
+                                                     __builtin_bit_cast(bf16x8, b[0]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[mt][1]),
+                                                     __builtin_bit_cast(bf16x8, b[1]), acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int o = 16 * mt + 4 * g + i;
+            const float z = __fadd_rn(acc[i], W.p[O_B2 + o]);
+            F.z2[mt][i] = z;
+            F.h2[mt][i] = silu(z);
+            part = __fadd_rn(part, __fmul_rn(F.h2[mt][i], bf2f(W.w3[o])));
+        }
+    }
+    // the four lane groups hold the partial sums of the same sample
+    const float p1 = __fadd_rn(part, __shfl_xor(part, 16));
+    const float p2 = __fadd_rn(p1, __shfl_xor(p1, 32));
+    F.y = __fadd_rn(p2, W.p[O_B3]);
+}
+
+__device__ inline void load_x(const float *__restrict__ feat, int64_t row, float *x)
+{
+    const float4 *r = reinterpret_cast<const float4 *>(feat + row * FSTRIDE);
+    const float4 a = r[0], b = r[1];
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y;
+}
+
+// y for every sample; mode 0 writes y, mode 1 writes exp(mu + sigma y)
+__global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__restrict__ feat, int64_t M, int mode,
+                                                     float mu, float sigma, float *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    u32x4 a[4][2];
+    load_a(W, lane, a);
+    const int64_t groups = (M + 15) / 16;
+    for (int64_t gi = wave; gi < groups; gi += nwaves) {
+        const int64_t s = gi * 16 + (lane & 15);
+        float x[NF];
+        load_x(feat, s < M ? s : M - 1, x);
+        fwd_state F;
+        mlp_forward16(W, a, x, lane, F);
+        if (s < M && lane < 16) out[s] = mode == 0 ? F.y : __expf(__fadd_rn(mu, __fmul_rn(sigma, F.y)));
+    }
+}
+
+// Training pass over a batch: forward, MSE loss against t (normalised
+// targets), back-propagation; each wave writes its gradient partial
+// (NPARAM floats) and loss partial to its slot.  Every wave runs the same
+// number of 16-sample groups (groups past the batch are masked), so the
+// block barriers match.
+constexpr int TB_WAVES = 4;
+struct wave_lds {
+    float x[16][NF];
+    float h1[16][H];
+    float h2[16][H];
+    float dz2[16][H];
+    float dz1[16][H];
+    float dy[16];
+};
+
+__global__ void __launch_bounds__(256) k_mlp_backward(mlp_dev W, const float *__restrict__ feat,
+                                                      const float *__restrict__ targ, const int64_t *__restrict__ idx,
+                                                      int64_t B, int iters, float *__restrict__ partial)
+{
+    __shared__ wave_lds L[TB_WAVES];
+    __shared__ float w2f[H][H + 1];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)blockIdx.x * TB_WAVES + wv;
+    const int64_t nwaves = (int64_t)gridDim.x * TB_WAVES;
+    for (int q = threadIdx.x; q < H * H; q += blockDim.x) w2f[q / H][q % H] = bf2f(W.w2[q]);
+    u32x4 a[4][2];
+    load_a(W, lane, a);
+    __syncthreads();
+
+    const int g = lane >> 4, sl = lane & 15;
+    float gW2[H];
+#pragma unroll
+    for (int o = 0; o < H; ++o) gW2[o] = 0.f;
+    float gW1[NF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, gl = 0.f;
+    const float inv2B = 2.0f / (float)B;
+    wave_lds &S = L[wv];
+
+    for (int it = 0; it < iters; ++it) {
+        const int64_t gi = wave + (int64_t)it * nwaves;
+        const int64_t b = gi * 16 + sl;
+        const bool valid = b < B;
+        const int64_t row = valid ? (idx ? idx[b] : b) : (idx ? idx[0] : 0);
+        float x[NF];
+        load_x(feat, row, x);
+        fwd_state F;
+        mlp_forward16(W, a, x, lane, F);
+        const float r = valid ? __fsub_rn(F.y, targ[row]) : 0.f;
+        const float dy = __fmul_rn(inv2B, r);
+        // dz2 for the lane's outputs; stage the per-sample rows in LDS
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o = 16 * mt + 4 * g + i;
+                S.h2[sl][o] = F.h2[mt][i];
+                S.dz2[sl][o] = __fmul_rn(__fmul_rn(dy, bf2f(W.w3[o])), dsilu(F.z2[mt][i]));
+            }
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) S.h1[sl][32 * ks + 8 * g + j] = F.h1[ks][j];
+        if (g == 0) {
+#pragma unroll
+            for (int i = 0; i < NF; ++i) S.x[sl][i] = x[i];
+            S.dy[sl] = dy;
+            gl = __fadd_rn(gl, __fmul_rn(r, r));
+        }
+        __syncthreads();
+        // dh1 = dz2 W2^T for the lane's units, then dz1
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 32 * ks + 8 * g + j;
+                float dh = 0.f;
+                for (int o = 0; o < H; ++o) dh = __fadd_rn(dh, __fmul_rn(S.dz2[sl][o], w2f[k][o]));
+                S.dz1[sl][k] = __fmul_rn(dh, dsilu(F.z1[ks][j]));
+            }
+        __syncthreads();
+        // lane l owns row l of W2 (input unit l), column l of W1, and index l
+        // of b1, b2, w3
+        for (int s2 = 0; s2 < 16; ++s2) {
+            const float h1s = S.h1[s2][lane];
+#pragma unroll
+            for (int o = 0; o < H; ++o) gW2[o] = __fadd_rn(gW2[o], __fmul_rn(h1s, S.dz2[s2][o]));
+            const float d1 = S.dz1[s2][lane];
+#pragma unroll
+            for (int i = 0; i < NF; ++i) gW1[i] = __fadd_rn(gW1[i], __fmul_rn(S.x[s2][i], d1));
+            gb1 = __fadd_rn(gb1, d1);
+            gb2 = __fadd_rn(gb2, S.dz2[s2][lane]);
+            gw3 = __fadd_rn(gw3, __fmul_rn(S.h2[s2][lane], S.dy[s2]));
+            gb3 = __fadd_rn(gb3, S.dy[s2]);
+        }
+        __syncthreads();
+    }
+    // loss partial: lanes 0..15 hold one sample each
+    float l1 = gl;
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) l1 = __fadd_rn(l1, __shfl_xor(l1, m));
+    float *P = partial + (size_t)wave * SLOT;
+#pragma unroll
+    for (int i = 0; i < NF; ++i) P[O_W1 + i * H + lane] = gW1[i];
+    P[O_B1 + lane] = gb1;
+#pragma unroll
+    for (int o = 0; o < H; ++o) P[O_W2 + lane * H + o] = gW2[o];
+    P[O_B2 + lane] = gb2;
+    P[O_W3 + lane] = gw3;
+    if (lane == 0) {
+        P[O_B3] = gb3;
+        P[NPARAM] = l1;
+    }
+}
+
+__global__ void k_reduce_slots(const float *__restrict__ partial, int64_t nslots, float *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= SLOT) return;
+    float a = 0.f;
+    for (int64_t w = 0; w < nslots; ++w) a = __fadd_rn(a, partial[(size_t)w * SLOT + p]);
+    out[p] = a;
+}
+
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v, const float *__restrict__ g,
+                       float lr, float b1, float b2, float eps, float c1, float c2, uint16_t *__restrict__ w1,
+                       uint16_t *__restrict__ w2, uint16_t *__restrict__ w3)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (i < O_B1) w1[i - O_W1] = (uint16_t)f2bf(pi);
+    else if (i >= O_W2 && i < O_B2) w2[i - O_W2] = (uint16_t)f2bf(pi);
+    else if (i >= O_W3 && i < O_B3) w3[i - O_W3] = (uint16_t)f2bf(pi);
+}
+
+__global__ void k_refresh_bf16(const float *__restrict__ p, uint16_t *__restrict__ w1, uint16_t *__restrict__ w2,
+                               uint16_t *__restrict__ w3)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    const float pi = p[i];
+    if (i < O_B1) w1[i - O_W1] = (uint16_t)f2bf(pi);
+    else if (i >= O_W2 && i < O_B2) w2[i - O_W2] = (uint16_t)f2bf(pi);
+    else if (i >= O_W3 && i < O_B3) w3[i - O_W3] = (uint16_t)f2bf(pi);
+}
+
+}  // namespace
+
+struct fr_mlp {
+    fr_ctx *ctx = nullptr;
+    float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    uint16_t *w1 = nullptr, *w2 = nullptr, *w3 = nullptr;
+    float *partial = nullptr;
+    int64_t partial_slots = 0;
+    int64_t step = 0;
+    float mu = 0.f, sigma = 1.f;
+};
+
+namespace {
+mlp_dev view(const fr_mlp *m) { return mlp_dev{m->p, m->w1, m->w2, m->w3}; }
+}
+
+extern "C" int fr_mlp_param_count(void) { return NPARAM; }
+
+extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev)
+{
+    if (!ctx || !s || !feat_dev) return FR_ERR_INVALID;
+    FR_CHECK(ctx, s->has_traj, FR_ERR_STATE, "trajectory not set");
+    FR_CHECK(ctx, s->frames == 0, FR_ERR_UNSUPPORTED, "features of batched frames: compute them per frame");
+    FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
+    const size_t cells = fr_cells(s->dim, N);
+    uint32_t *hist = nullptr;
+    fr_status st = fr_alloc(ctx, (void **)&hist, cells * sizeof(uint32_t));
+    if (st) return st;
+    hipStream_t q = ctx->stream;
+    hipError_t e = hipMemsetAsync(hist, 0, cells * sizeof(uint32_t), q);
+    if (e == hipSuccess && s->M > 0) {
+        hipLaunchKernelGGL(k_cell_hist, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist);
+        const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
+        for (int ax = 0; ax < s->dim; ++ax)
+            hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, hist, (int)N, s->dim, ax);
+        hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist,
+                           feat_dev);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(q);
+    hipFree(hist);
+    FR_CHECK(ctx, e == hipSuccess, FR_ERR_HIP, "features: %s", hipGetErrorString(e));
+    return FR_OK;
+}
+
+extern "C" fr_status fr_mlp_create(fr_ctx *ctx, fr_mlp **out)
+{
+    if (!ctx || !out) return FR_ERR_INVALID;
+    fr_mlp *m = new fr_mlp;
+    m->ctx = ctx;
+    fr_status st;
+    if ((st = fr_alloc(ctx, (void **)&m->p, NPARAM * 4)) || (st = fr_alloc(ctx, (void **)&m->m, NPARAM * 4)) ||
+        (st = fr_alloc(ctx, (void **)&m->v, NPARAM * 4)) || (st = fr_alloc(ctx, (void **)&m->grad, SLOT * 4)) ||
+        (st = fr_alloc(ctx, (void **)&m->w1, NF * H * 2)) || (st = fr_alloc(ctx, (void **)&m->w2, H * H * 2)) ||
+        (st = fr_alloc(ctx, (void **)&m->w3, H * 2))) {
+        fr_mlp_destroy(m);
+        return st;
+    }
+    FR_HIP(ctx, hipMemsetAsync(m->p, 0, NPARAM * 4, ctx->stream));
+    FR_HIP(ctx, hipMemsetAsync(m->m, 0, NPARAM * 4, ctx->stream));
+    FR_HIP(ctx, hipMemsetAsync(m->v, 0, NPARAM * 4, ctx->stream));
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblk(NPARAM, 256)), dim3(256), 0, ctx->stream, m->p, m->w1, m->w2, m->w3);
+    FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = m;
+    return FR_OK;
+}
+
+extern "C" void fr_mlp_destroy(fr_mlp *m)
+{
+    if (!m) return;
+    if (m->ctx) hipStreamSynchronize(m->ctx->stream);
+    hipFree(m->p); hipFree(m->m); hipFree(m->v); hipFree(m->grad);
+    hipFree(m->w1); hipFree(m->w2); hipFree(m->w3); hipFree(m->partial);
+    delete m;
+}
+
+extern "C" fr_status fr_mlp_set_params(fr_mlp *m, const float *params, float mu, float sigma)
+{
+    if (!m || !params) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    FR_CHECK(ctx, std::isfinite(mu) && std::isfinite(sigma) && sigma > 0.f, FR_ERR_INVALID,
+             "output normalisation must be finite with sigma > 0");
+    FR_HIP(ctx, hipMemcpyAsync(m->p, params, NPARAM * 4, hipMemcpyHostToDevice, ctx->stream));
+    FR_HIP(ctx, hipMemsetAsync(m->m, 0, NPARAM * 4, ctx->stream));
+    FR_HIP(ctx, hipMemsetAsync(m->v, 0, NPARAM * 4, ctx->stream));
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(nblk(NPARAM, 256)), dim3(256), 0, ctx->stream, m->p, m->w1, m->w2, m->w3);
+    FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    m->mu = mu;
+    m->sigma = sigma;
+    m->step = 0;
+    return FR_OK;
+}
+
+extern "C" fr_status fr_mlp_get_params(fr_mlp *m, float *params, float *mu, float *sigma)
+{
+    if (!m || !params) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    FR_HIP(ctx, hipMemcpyAsync(params, m->p, NPARAM * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (mu) *mu = m->mu;
+    if (sigma) *sigma = m->sigma;
+    return FR_OK;
+}
+
+static unsigned fwd_blocks(fr_ctx *ctx, int64_t M)
+{
+    const int64_t groups = (M + 15) / 16;
+    const int64_t waves = std::min<int64_t>(groups, (int64_t)std::max(ctx->cu_count, 1) * 32);
+    return (unsigned)std::max<int64_t>(1, (waves + 3) / 4);
+}
+
+extern "C" fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M, float *y_dev)
+{
+    if (!m || !feat_dev || !y_dev || M < 0) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    if (M == 0) return FR_OK;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, M)), dim3(256), 0, ctx->stream, view(m), feat_dev, M, 0,
+                       0.f, 1.f, y_dev);
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s)
+{
+    if (!m || !feat_dev || !s) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    if (s->M > 0)
+        hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, s->M)), dim3(256), 0, ctx->stream, view(m), feat_dev,
+                           s->M, 1, m->mu, m->sigma, s->dcf);
+    FR_HIP(ctx, hipGetLastError());
+    s->has_dcf = true;
+    return FR_OK;
+}
+
+// gradient (and loss) of the batch into m->grad; shared by train and grad
+static fr_status mlp_backward(fr_mlp *m, const float *feat_dev, const float *targ_dev, const int64_t *idx_dev,
+                              int64_t B)
+{
+    fr_ctx *ctx = m->ctx;
+    FR_CHECK(ctx, B > 0 && feat_dev && targ_dev, FR_ERR_INVALID, "empty training batch");
+    const int64_t groups = (B + 15) / 16;
+    const int64_t maxw = (int64_t)std::max(ctx->cu_count, 1) * 4;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((groups + TB_WAVES - 1) / TB_WAVES, maxw));
+    const int64_t nwaves = blocks * TB_WAVES;
+    const int iters = (int)((groups + nwaves - 1) / nwaves);
+    if (nwaves > m->partial_slots) {
+        FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        hipFree(m->partial);
+        m->partial = nullptr;
+        m->partial_slots = 0;
+        fr_status st = fr_alloc(ctx, (void **)&m->partial, (size_t)nwaves * SLOT * 4);
+        if (st) return st;
+        m->partial_slots = nwaves;
+    }
+    hipLaunchKernelGGL(k_mlp_backward, dim3((unsigned)blocks), dim3(64 * TB_WAVES), 0, ctx->stream, view(m), feat_dev,
+                       targ_dev, idx_dev, B, iters, m->partial);
+    hipLaunchKernelGGL(k_reduce_slots, dim3(nblk(SLOT, 256)), dim3(256), 0, ctx->stream, m->partial, nwaves, m->grad);
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_mlp_grad(fr_mlp *m, const float *feat_dev, const float *targ_dev, const int64_t *idx_dev,
+                                 int64_t B, float *grad_host, float *loss)
+{
+    if (!m || !grad_host) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    fr_status st = mlp_backward(m, feat_dev, targ_dev, idx_dev, B);
+    if (st) return st;
+    float tmp[SLOT];
+    FR_HIP(ctx, hipMemcpyAsync(tmp, m->grad, SLOT * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(grad_host, tmp, NPARAM * 4);
+    if (loss) *loss = tmp[NPARAM] / (float)B;
+    return FR_OK;
+}
+
+extern "C" fr_status fr_mlp_train_step(fr_mlp *m, const float *feat_dev, const float *targ_dev,
+                                       const int64_t *idx_dev, int64_t B, float lr, float *loss)
+{
+    if (!m) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    FR_CHECK(ctx, lr > 0.f && std::isfinite(lr), FR_ERR_INVALID, "learning rate must be positive");
+    fr_status st = mlp_backward(m, feat_dev, targ_dev, idx_dev, B);
+    if (st) return st;
+    m->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM, 256)), dim3(256), 0, ctx->stream, m->p, m->m, m->v, m->grad, lr, b1,
+                       b2, eps, c1, c2, m->w1, m->w2, m->w3);
+    FR_HIP(ctx, hipGetLastError());
+    if (loss) {
+        float l;
+        FR_HIP(ctx, hipMemcpyAsync(&l, m->grad + NPARAM, 4, hipMemcpyDeviceToHost, ctx->stream));
+        FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        *loss = l / (float)B;
+    }
+    return FR_OK;
+}

@@ -34,6 +34,7 @@ struct gctx {
     int L;
     int dim;
     int N;
+    int frames;    // batched 2D frames along z (0: one grid)
 };
 
 __device__ __forceinline__ size_t cell_index(int x, int y, int z, int N)
@@ -47,7 +48,7 @@ __global__ void k_count_cells(const float *__restrict__ traj, int64_t M, gctx g,
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     fr_box b;
-    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b, g.frames);
     fr_box_taps(b, b.s, b.e, g.hw, g.L, [&](int x, int y, int z, int) {
         atomicAdd(&cnt[cell_index(x, y, z, g.N)], 1u);
     });
@@ -81,7 +82,7 @@ __global__ void k_count_records(const float *__restrict__ traj, int64_t M, gctx 
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     fr_box b;
-    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b, g.frames);
     uint32_t r = 0;
     if (b.s[0] <= b.e[0] && b.s[1] <= b.e[1] && b.s[2] <= b.e[2])
         for_each_unit(b, g.N, cell_unit, units, [&](int, const fr_unit &, const int *lo, const int *hi) {
@@ -100,7 +101,7 @@ __global__ void k_emit_records(const float *__restrict__ traj, int64_t M, gctx g
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     fr_box b;
-    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b, g.frames);
     uint64_t p = pos[s];
     if (b.s[0] <= b.e[0] && b.s[1] <= b.e[1] && b.s[2] <= b.e[2])
         for_each_unit(b, g.N, cell_unit, units, [&](int u, const fr_unit &, const int *lo, const int *hi) {
@@ -172,7 +173,7 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
     const uint32_t np = padded_taps(n);
     H[r] = make_uint2((uint32_t)s, np);
     fr_box b;
-    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b);
+    fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b, g.frames);
     uint2 *w = T + toff[r];
     uint32_t i = 0;
     for_each_unit(b, g.N, cell_unit, units, [&](int u, const fr_unit &U, const int *lo, const int *hi) {
@@ -371,9 +372,9 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     p->exact_order = tune ? (tune->exact_order != 0) : 0;
 
     const int N = (int)p->N;
-    const size_t cells = fr_cells(p->dim, p->N);
+    const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int64_t M = s->M;
-    gctx g{p->kHW, p->L, p->dim, N};
+    gctx g{p->kHW, p->L, p->dim, N, p->frames};
     fr_status e;
 
     // 1. per-cell tap histogram
@@ -397,7 +398,7 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     const uint64_t space_thr = p->exact_order ? thr : ~uint64_t(0);
     const uint64_t slice_thr = p->exact_order ? ~uint64_t(0) : thr;
     unit_builder ub{cnt, N, p->dim, space_thr, {}};
-    const int nz = p->dim == 3 ? N : 1;
+    const int nz = p->frames > 0 ? p->frames : (p->dim == 3 ? N : 1);
     for (int oz = 0; oz < nz; oz += geom.bz)
         for (int oy = 0; oy < N; oy += geom.by)
             for (int ox = 0; ox < N; ox += geom.bx) {
@@ -411,9 +412,15 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     // (longest first), then the rest along a Morton curve of their origins so
     // that items running at the same time share samples in cache.
     const uint64_t heavy_cut = thr / 4;
+    // (batched frames share no samples: frame-major, Morton within a frame)
     auto morton = [&](const host_unit &u) {
         const uint32_t a = (uint32_t)u.ox / geom.bx, b = (uint32_t)u.oy / geom.by, c = (uint32_t)u.oz / geom.bz;
         uint64_t m = 0;
+        if (p->frames > 0) {
+            for (int bit = 0; bit < 21; ++bit)
+                m |= ((uint64_t)((a >> bit) & 1) << (2 * bit)) | ((uint64_t)((b >> bit) & 1) << (2 * bit + 1));
+            return ((uint64_t)c << 42) | m;
+        }
         for (int bit = 0; bit < 21; ++bit)
             m |= ((uint64_t)((a >> bit) & 1) << (3 * bit)) | ((uint64_t)((b >> bit) & 1) << (3 * bit + 1)) |
                  ((uint64_t)((c >> bit) & 1) << (3 * bit + 2));
@@ -599,11 +606,13 @@ extern "C" fr_status fr_plan_create(fr_ctx *ctx, const fr_samples *s, unsigned N
     FR_CHECK(ctx, N >= 2 && N <= 4096, FR_ERR_INVALID, "grid size %u outside [2,4096]", N);
     FR_CHECK(ctx, s->dim == 2 || (uint64_t)N * N * N < (uint64_t(1) << 31), FR_ERR_UNSUPPORTED,
              "3D grid %u^3 exceeds 2^31 cells", N);
+    FR_CHECK(ctx, s->frames == 0 || (uint64_t)s->frames * N * N < (uint64_t(1) << 31), FR_ERR_UNSUPPORTED,
+             "%d frames of %u^2 exceed 2^31 cells", s->frames, N);
     FR_CHECK(ctx, L >= 2 && L <= (1 << 20), FR_ERR_INVALID, "LUT length %d invalid", L);
     FR_CHECK(ctx, kWidth > 0.f && kWidth <= 16.f, FR_ERR_INVALID, "kernel width %g invalid", (double)kWidth);
     fr_plan *p = new (std::nothrow) fr_plan;
     if (!p) return FR_ERR_NOMEM;
-    p->ctx = ctx; p->dim = s->dim; p->N = N; p->M = s->M;
+    p->ctx = ctx; p->dim = s->dim; p->N = N; p->M = s->M; p->frames = s->frames;
     p->kWidth = kWidth; p->kHW = kWidth / 2.0f; p->L = L;     // GridLut.cpp:86
     p->host_lut.assign(lut, lut + L);
     fr_status e = fr_alloc(ctx, (void **)&p->lut, sizeof(float) * L);

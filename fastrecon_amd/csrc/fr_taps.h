@@ -21,8 +21,11 @@ struct fr_box {
     float c[3];
 };
 
+// For batched 2D frames (frames > 0) the third trajectory row holds the
+// sample's frame index f: the box's z range is exactly [f, f] and its z
+// centre is f, so dz = 0 and the taps are the 2D ones of that frame.
 __device__ __forceinline__ void fr_sample_box(int dim, int N, const float *__restrict__ traj, int64_t M,
-                                              int64_t s, float hw, fr_box &b)
+                                              int64_t s, float hw, fr_box &b, int frames = 0)
 {
 #pragma unroll
     for (int i = 0; i < 3; ++i) { b.c[i] = 0.f; b.s[i] = 0; b.e[i] = 0; }
@@ -33,6 +36,13 @@ __device__ __forceinline__ void fr_sample_box(int dim, int N, const float *__res
         int en = (int)floorf(c + hw);
         b.s[i] = st < 0 ? 0 : st;
         b.e[i] = en > N - 1 ? N - 1 : en;
+    }
+    if (frames > 0) {
+        const float f = traj[(int64_t)2 * M + s];
+        const int fi = min(max((int)f, 0), frames - 1);
+        b.c[2] = (float)fi;
+        b.s[2] = fi;
+        b.e[2] = fi;
     }
 }
 

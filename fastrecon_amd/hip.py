@@ -24,12 +24,15 @@ EXPORTS = [
     "fr_samples_set_coil", "fr_samples_update", "fr_samples_set_data_device", "fr_samples_set_dcf_device",
     "fr_samples_set_traj_device", "fr_samples_get_dcf",
     "fr_plan_create", "fr_plan_destroy", "fr_plan_get_info",
-    "fr_grids_create", "fr_grids_destroy", "fr_grids_download", "fr_grids_upload", "fr_grids_device_ptr",
+    "fr_grids_create", "fr_grids_create_frames", "fr_samples_create_frames", "fr_grids_destroy", "fr_grids_download", "fr_grids_upload", "fr_grids_device_ptr",
     "fr_grid_execute", "fr_degrid_execute",
     "fr_fft_create", "fr_fft_destroy", "fr_fft_execute",
     "fr_fftshift", "fr_sos", "fr_deapod_map", "fr_lowfilter", "fr_normalize", "fr_crop", "fr_sos_map",
     "fr_grids_copy",
     "fr_dcf_pipe_menon",
+    "fr_graph_create", "fr_graph_launch", "fr_graph_destroy", "fr_autotune",
+    "fr_ldcf_features", "fr_mlp_param_count", "fr_mlp_create", "fr_mlp_destroy", "fr_mlp_set_params",
+    "fr_mlp_get_params", "fr_mlp_forward", "fr_mlp_dcf", "fr_mlp_grad", "fr_mlp_train_step",
     "fr_malloc", "fr_free", "fr_memcpy_d2h", "fr_memcpy_h2d",
 ]
 
@@ -86,6 +89,12 @@ def load():
         "fr_plan_destroy": [_vp],
         "fr_plan_get_info": [_vp, C.POINTER(fr_plan_info)],
         "fr_grids_create": [_vp, C.c_int, C.c_uint, C.c_int, _pp],
+        "fr_grids_create_frames": [_vp, C.c_uint, C.c_int, C.c_int, _pp],
+        "fr_samples_create_frames": [_vp, C.c_int, _vp, C.c_int, C.c_int, _pp],
+        "fr_graph_create": [_vp, _vp, _vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp, _pp],
+        "fr_graph_launch": [_vp],
+        "fr_autotune": [_vp, _vp, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_int, C.c_int, C.POINTER(fr_tune),
+                        C.POINTER(C.c_float), _vp],
         "fr_grids_destroy": [_vp],
         "fr_grids_download": [_vp, C.c_int, _vp],
         "fr_grids_upload": [_vp, C.c_int, _vp],
@@ -103,6 +112,15 @@ def load():
         "fr_crop": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
         "fr_sos_map": [_vp, _vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp],
         "fr_grids_copy": [_vp, _vp, _vp],
+        "fr_ldcf_features": [_vp, _vp, C.c_uint, _vp],
+        "fr_mlp_param_count": [],
+        "fr_mlp_create": [_vp, _pp],
+        "fr_mlp_set_params": [_vp, _vp, C.c_float, C.c_float],
+        "fr_mlp_get_params": [_vp, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "fr_mlp_forward": [_vp, _vp, C.c_int64, _vp],
+        "fr_mlp_dcf": [_vp, _vp, _vp],
+        "fr_mlp_grad": [_vp, _vp, _vp, _vp, C.c_int64, _vp, C.POINTER(C.c_float)],
+        "fr_mlp_train_step": [_vp, _vp, _vp, _vp, C.c_int64, C.c_float, C.POINTER(C.c_float)],
         "fr_malloc": [_vp, C.c_size_t, _pp],
         "fr_free": [_vp, _vp],
         "fr_memcpy_d2h": [_vp, _vp, _vp, C.c_size_t],
@@ -118,6 +136,10 @@ def load():
     L.fr_ctx_stream.restype = _vp
     L.fr_grids_device_ptr.argtypes = [_vp]
     L.fr_grids_device_ptr.restype = _vp
+    L.fr_mlp_destroy.argtypes = [_vp]
+    L.fr_mlp_destroy.restype = None
+    L.fr_graph_destroy.argtypes = [_vp]
+    L.fr_graph_destroy.restype = None
     _lib = L
     return L
 
@@ -188,10 +210,20 @@ class Context:
 
 
 class Samples:
-    def __init__(self, ctx, dim, M, coils, sample_type=FR_SAMPLES_C64):
+    def __init__(self, ctx, dim, M, coils, sample_type=FR_SAMPLES_C64, frame_samples=None):
+        """frame_samples: per-frame sample counts for batched 2D frames (dim 2; M is their sum)."""
         self.ctx, self.dim, self.M, self.C = ctx, dim, int(M), int(coils)
+        self.frames = 0
         h = _vp()
-        ctx.check(ctx.L.fr_samples_create(ctx.h, dim, self.M, self.C, sample_type, C.byref(h)), "fr_samples_create")
+        if frame_samples is not None:
+            fs = np.ascontiguousarray(frame_samples, np.int64)
+            assert dim == 2 and int(fs.sum()) == self.M
+            self.frames = len(fs)
+            ctx.check(ctx.L.fr_samples_create_frames(ctx.h, self.frames, _ptr(fs), self.C, sample_type, C.byref(h)),
+                      "fr_samples_create_frames")
+        else:
+            ctx.check(ctx.L.fr_samples_create(ctx.h, dim, self.M, self.C, sample_type, C.byref(h)),
+                      "fr_samples_create")
         self.h = h
         ctx.adopt(self)
 
@@ -275,11 +307,17 @@ class Plan:
 
 
 class Grids:
-    def __init__(self, ctx, dim, N, coils):
-        self.ctx, self.dim, self.N, self.C = ctx, dim, int(N), int(coils)
-        self.cells = self.N ** dim
+    def __init__(self, ctx, dim, N, coils, frames=0):
+        """frames > 0: batched 2D frames, [coil][frame][N^2]."""
+        self.ctx, self.dim, self.N, self.C, self.frames = ctx, dim, int(N), int(coils), int(frames)
+        self.cells = self.N ** dim * max(self.frames, 1)
         h = _vp()
-        ctx.check(ctx.L.fr_grids_create(ctx.h, dim, self.N, self.C, C.byref(h)), "fr_grids_create")
+        if self.frames:
+            assert dim == 2
+            ctx.check(ctx.L.fr_grids_create_frames(ctx.h, self.N, self.C, self.frames, C.byref(h)),
+                      "fr_grids_create_frames")
+        else:
+            ctx.check(ctx.L.fr_grids_create(ctx.h, dim, self.N, self.C, C.byref(h)), "fr_grids_create")
         self.h = h
         ctx.adopt(self)
 
@@ -366,6 +404,14 @@ class DeviceBuffer:
             pass
 
 
+class DevicePtr:
+    """A byte offset into a DeviceBuffer, usable wherever a buffer's .ptr is read."""
+
+    def __init__(self, buf, offset):
+        assert 0 <= offset <= buf.nbytes
+        self.buf, self.ptr = buf, buf.ptr + int(offset)
+
+
 # ----------------------------------------------------------------- calls
 def grid_execute(plan, samples, grids):
     plan.ctx.check(plan.ctx.L.fr_grid_execute(plan.h, samples.h, grids.h), "fr_grid_execute")
@@ -437,3 +483,114 @@ def sos_map(ctx, grids, cmap, rc, shift=True):
 
 def grids_copy(ctx, src, dst):
     ctx.check(ctx.L.fr_grids_copy(ctx.h, src.h, dst.h), "fr_grids_copy")
+
+
+
+# ------------------------------------------------------ learned density compensation
+def ldcf_features(ctx, samples, N):
+    """Per-sample feature rows (M x 8 floats, device) of the samples' normalised trajectory."""
+    out = DeviceBuffer(ctx, 32 * max(samples.M, 1))
+    ctx.check(ctx.L.fr_ldcf_features(ctx.h, samples.h, int(N), _vp(out.ptr)), "fr_ldcf_features")
+    return out
+
+
+class MLP:
+    """The 6-64-64-1 SiLU network (BF16 weight matrices, FP32 master copy and Adam state on the device)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = _vp()
+        ctx.check(ctx.L.fr_mlp_create(ctx.h, C.byref(h)), "fr_mlp_create")
+        self.h = h
+        self.nparam = ctx.L.fr_mlp_param_count()
+        ctx.adopt(self)
+
+    def set_params(self, params, mu=0.0, sigma=1.0):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.nparam
+        self.ctx.check(self.ctx.L.fr_mlp_set_params(self.h, _ptr(p), float(mu), float(sigma)), "fr_mlp_set_params")
+
+    def get_params(self):
+        p = np.zeros(self.nparam, np.float32)
+        mu, sg = C.c_float(), C.c_float()
+        self.ctx.check(self.ctx.L.fr_mlp_get_params(self.h, _ptr(p), C.byref(mu), C.byref(sg)), "fr_mlp_get_params")
+        return p, mu.value, sg.value
+
+    def forward(self, feat, M):
+        y = DeviceBuffer(self.ctx, 4 * max(M, 1))
+        self.ctx.check(self.ctx.L.fr_mlp_forward(self.h, _vp(feat.ptr), int(M), _vp(y.ptr)), "fr_mlp_forward")
+        return y
+
+    def dcf(self, feat, samples):
+        self.ctx.check(self.ctx.L.fr_mlp_dcf(self.h, _vp(feat.ptr), samples.h), "fr_mlp_dcf")
+
+    def grad(self, feat, targ, B, idx=None):
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.ctx.L.fr_mlp_grad(self.h, _vp(feat.ptr), _vp(targ.ptr), _vp(idx.ptr) if idx else None,
+                                              int(B), _ptr(g), C.byref(loss)), "fr_mlp_grad")
+        return g, loss.value
+
+    def train_step(self, feat, targ, B, lr, idx=None, want_loss=True):
+        loss = C.c_float()
+        self.ctx.check(self.ctx.L.fr_mlp_train_step(self.h, _vp(feat.ptr), _vp(targ.ptr),
+                                                    _vp(idx.ptr) if idx else None, int(B), float(lr),
+                                                    C.byref(loss) if want_loss else None), "fr_mlp_train_step")
+        return loss.value if want_loss else None
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fr_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+
+# ------------------------------------------------------------ graphs, autotuning
+class Graph:
+    """One captured reconstruction pass (gridding [+ FFT [+ shift/crop/SOS]])."""
+
+    def __init__(self, plan, samples, grids, fft=None, rc=(0, 0, 0), shift=True, deapod=None, out=None):
+        self.ctx = plan.ctx
+        rcx, rcy, rcz = rc
+        h = _vp()
+        self.ctx.check(self.ctx.L.fr_graph_create(plan.h, samples.h, grids.h, fft.h if fft else None, rcx, rcy, rcz,
+                                                  1 if shift else 0, _vp(deapod.ptr) if deapod else None,
+                                                  _vp(out.ptr) if out else None, C.byref(h)), "fr_graph_create")
+        self.h = h
+        self.ctx.adopt(self)
+
+    def launch(self):
+        self.ctx.check(self.ctx.L.fr_graph_launch(self.h), "fr_graph_launch")
+
+    def close(self):
+        if self.h:
+            self.ctx.L.fr_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx, samples, N, kwidth, lut, candidates=None, reps=3):
+    """Fastest fr_tune (as a dict) for these samples; also each candidate's ms."""
+    lut = np.ascontiguousarray(lut, np.float32)
+    cands = None
+    n = 0
+    if candidates:
+        n = len(candidates)
+        cands = (fr_tune * n)(*[fr_tune(**{k: int(v) for k, v in c.items()}) for c in candidates])
+    best = fr_tune()
+    best_ms = C.c_float()
+    ms = np.zeros(max(n, 6), np.float32)
+    ctx.check(ctx.L.fr_autotune(ctx.h, samples.h, int(N), float(kwidth), _ptr(lut), len(lut), cands, n, int(reps),
+                                C.byref(best), C.byref(best_ms), _ptr(ms)), "fr_autotune")
+    return {k: getattr(best, k) for k, _ in fr_tune._fields_}, best_ms.value, ms

@@ -89,6 +89,63 @@ def kooshball3d(spokes=51200, readout=256):
     return traj, dcf
 
 
+# ------------------------------------------- learned-DCF training trajectories
+def random_rotation(seed):
+    """A seeded rotation matrix (QR of a Gaussian matrix, sign-fixed)."""
+    g = complex_gaussian(9, seed).real.reshape(3, 3)
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diag(r))[None, :]
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def radial3d_vd(spokes, readout, alpha=1.0, seed=0):
+    """Full-diameter spokes on rotated Fibonacci directions with variable
+    density along the spoke: k = 0.5 sign(s) |s|^alpha u, s = (j - R/2)/(R/2)."""
+    u = fibonacci_dirs(spokes) @ random_rotation(seed).T
+    s = (np.arange(readout, dtype=np.float64) - readout / 2) / (readout / 2)
+    r = 0.5 * np.sign(s) * np.abs(s) ** alpha
+    k = (r[None, :, None] * u[:, None, :]).reshape(-1, 3)
+    return k.astype(np.float32)
+
+
+def cones3d(cones, interleaves, readout, turns=4.0, alpha=1.0, seed=0):
+    """Centre-out 3D cones: cone c has polar angle (c + 1/2) pi / cones and
+    `interleaves` spirals k = 0.5 t^a (sin th cos ph, sin th sin ph, cos th),
+    ph = 2 pi turns t^a + 2 pi i / interleaves + (seeded cone offset)."""
+    t = np.arange(readout, dtype=np.float64) / readout
+    ta = t ** alpha
+    off = uniform(np.arange(cones, dtype=np.uint64), seed) * 2.0 * math.pi
+    ks = []
+    for c in range(cones):
+        th = (c + 0.5) * math.pi / cones
+        for i in range(interleaves):
+            ph = 2.0 * math.pi * turns * ta + 2.0 * math.pi * i / interleaves + off[c]
+            ks.append(np.stack([0.5 * ta * math.sin(th) * np.cos(ph), 0.5 * ta * math.sin(th) * np.sin(ph),
+                                0.5 * ta * math.cos(th) * np.ones_like(ta)], axis=1))
+    return np.concatenate(ks, 0).astype(np.float32)
+
+
+def ldcf_training_trajectories(count=200, seed=7, scale=1.0):
+    """The seeded training set: variable-density 3D radial and cones
+    trajectories for a 128^3 image (256^3 grid).  `scale` shrinks the spoke /
+    interleave counts (tests use small sets).  Yields (name, k[M,3])."""
+    for n in range(count):
+        r = uniform(np.arange(4, dtype=np.uint64) + 4 * n, seed)
+        if n % 2 == 0:
+            spokes = max(16, int(scale * (4000 + r[0] * 56000)))
+            readout = int(128 + 128 * (r[1] > 0.5))
+            alpha = 1.0 + 1.0 * r[2]
+            yield f"radial{n}", radial3d_vd(spokes, readout, alpha, seed=seed * 1000 + n)
+        else:
+            cones = max(4, int(scale * (16 + r[0] * 48)))
+            inter = max(4, int(scale * (64 + r[1] * 448)))
+            alpha = 1.0 + 1.0 * r[2]
+            turns = 2.0 + 6.0 * r[3]
+            yield f"cones{n}", cones3d(cones, inter, 256, turns, alpha, seed=seed * 1000 + n)
+
+
 # ------------------------------------------------------- 2D spiral frames
 SPIRAL_ALPHA = 1.5
 SPIRAL_TURNS = 16.0
@@ -125,3 +182,10 @@ CONFIGS = {
 def grid_size(image, og):
     """GridLut::plan: N = unsigned(reconSize * og) (GridLut.cpp:45)."""
     return int(np.float32(image) * np.float32(og))
+
+
+def spiral_frames(frames=64, samples=4096, seed=4, lo=8, hi=48):
+    """The batched uneven 2D spiral series: frame f has a seeded number of
+    interleaves in [lo, hi].  Returns (list of (k, dcf) per frame, counts)."""
+    counts = spiral_frame_counts(frames, lo, hi, seed)
+    return [spiral2d(int(n), samples) for n in counts], counts

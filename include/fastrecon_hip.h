@@ -141,6 +141,17 @@ fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info);
 /* Replaces cuImageData (ImageData/cuImageData.h:7-36): C device-resident
  * grids of N^dim complex64, index x + N*y + N*N*z (GridLut.cpp:111). */
 fr_status fr_grids_create(fr_ctx *ctx, int dim, unsigned N, int coils, fr_grids **out);
+/* Batched 2D frames (net-new: the reference reconstructs a series as
+ * separate ReconData/GridLut instances that share one plan, GridLut.h:24-26
+ * setIndex, cuGridLut.h:50-56).  Frame f owns frame_samples[f] consecutive
+ * samples; fr_samples_set_traj takes the 2D points of all frames (each
+ * frame normalised with its own bounds, as its own ReconData would be).  A
+ * plan built on such samples grids every frame in one launch into grids from
+ * fr_grids_create_frames, laid out [coil][frame][N^2]; FFT plans for them use
+ * batch = coils * frames; fr_sos writes one rcx x rcy image per frame. */
+fr_status fr_samples_create_frames(fr_ctx *ctx, int frames, const int64_t *frame_samples, int coils,
+                                   int sample_type, fr_samples **out);
+fr_status fr_grids_create_frames(fr_ctx *ctx, unsigned N, int coils, int frames, fr_grids **out);
 fr_status fr_grids_destroy(fr_grids *g);
 fr_status fr_grids_download(const fr_grids *g, int coil, void *host);
 fr_status fr_grids_upload(fr_grids *g, int coil, const void *host);
@@ -203,6 +214,59 @@ fr_status fr_deapod_map(fr_ctx *ctx, int dim, unsigned N, float kWidth, const fl
  * starting from the samples' current DCF (or 1 if init_ones), written back
  * to the samples' DCF.  Uses this plan's gridding and degridding kernels. */
 fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones);
+
+/* ------------------------------------------------- graphs and autotuning
+ * Net-new (the reference launches per coil and partition with a device
+ * synchronisation after each, cuGridLut.cu:261-309).  fr_graph_create runs
+ * one eager pass (gridding; the FFT if f; shift/crop/SOS into out_dev if
+ * out_dev), then captures the same pass on the context's stream into a HIP
+ * graph; fr_graph_launch replays it on the current sample data. */
+typedef struct fr_graph fr_graph;
+fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *g, fr_fft *f, unsigned rcx, unsigned rcy,
+                          unsigned rcz, int apply_shift, const float *deapod_dev, float *out_dev, fr_graph **out);
+fr_status fr_graph_launch(fr_graph *gr);
+void fr_graph_destroy(fr_graph *gr);
+/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: unit
+ * shapes around the default) on these samples and returns the fastest in
+ * *best (and its time); cand_ms (optional, ncand or 6 entries) gets each
+ * candidate's time, -1 for candidates the plan refuses.  The LUT length is
+ * not tuned: tap weights are resolved when the plan is built, so L does not
+ * change the pass time, and L = 512 keeps reference parity. */
+fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, const float *lut, int L,
+                      const fr_tune *candidates, int ncand, int reps, fr_tune *best, float *best_ms,
+                      float *cand_ms);
+
+/* --------------------------------------------- learned density compensation
+ * Net-new (the reference has no DCF estimation).  Features per sample, 8
+ * floats per row (6 used): |u|, azimuth/pi, polar/pi, log1p(n_1)/8,
+ * log1p(n_2)/8, log1p(n_4)/8, with u = (t - (N-1)/2)/(N/2) for the samples'
+ * normalised trajectory t and n_r the number of other samples whose nearest
+ * cell lies within Chebyshev distance r cells.  feat_dev: M*8 floats. */
+fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev);
+
+/* MLP 6 -> 64 -> 64 -> 1 (SiLU), BF16 weight matrices, FP32 biases and
+ * accumulation.  Parameters are a flat FP32 vector of fr_mlp_param_count()
+ * values: W1[6][64], b1[64], W2[64][64] ([in][out]), b2[64], w3[64], b3.  The
+ * predicted DCF is exp(mu + sigma * y). */
+typedef struct fr_mlp fr_mlp;
+int fr_mlp_param_count(void);
+fr_status fr_mlp_create(fr_ctx *ctx, fr_mlp **out);
+void fr_mlp_destroy(fr_mlp *m);
+fr_status fr_mlp_set_params(fr_mlp *m, const float *params, float mu, float sigma);
+fr_status fr_mlp_get_params(fr_mlp *m, float *params, float *mu, float *sigma);
+/* y_dev[M] = network output for feature rows feat_dev[M*8]. */
+fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M, float *y_dev);
+/* The samples' DCF <- exp(mu + sigma * y) (feature rows of the same samples). */
+fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s);
+/* Mean squared error of y against targ_dev over the batch rows idx_dev[B]
+ * (or rows 0..B-1 if idx_dev is NULL), its gradient (host, param order) and
+ * the loss; deterministic. */
+fr_status fr_mlp_grad(fr_mlp *m, const float *feat_dev, const float *targ_dev, const int64_t *idx_dev, int64_t B,
+                      float *grad_host, float *loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on that loss; *loss (optional)
+ * is the batch loss before the step. */
+fr_status fr_mlp_train_step(fr_mlp *m, const float *feat_dev, const float *targ_dev, const int64_t *idx_dev,
+                            int64_t B, float lr, float *loss);
 
 /* --------------------------------------------------------- device memory */
 fr_status fr_malloc(fr_ctx *ctx, size_t bytes, void **dev);

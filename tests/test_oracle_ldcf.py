@@ -1,0 +1,58 @@
+"""Pins of the learned-DCF oracle (oracle/ldcf_oracle.py): the MLP gradient
+against central finite differences in FP64, the neighbour counts against an
+independent k-d tree (Chebyshev metric), the angular features against closed
+forms, and the BF16 rounding against known values."""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+import ldcf_oracle as lo
+
+
+def test_bf16_round_known_values():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1.0 + 2 ** -9, 65504.0], np.float32)
+    r = lo.bf16_round(v)
+    # ties go to even: 1 + 2^-8 is halfway between 1 and 1 + 2^-7 -> 1
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 2 ** -6) and r[3] == -2.5 and r[4] == 1.0
+    assert r[5] == 65536.0
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gradient_matches_finite_differences(seed):
+    rng = np.random.default_rng(seed)
+    p = lo.init_params(seed).astype(np.float64) + 0.05 * rng.standard_normal(lo.NPARAM)
+    x = rng.standard_normal((40, 6))
+    t = rng.standard_normal(40)
+    L, g = lo.loss_grad(p, x, t, emulate=False)
+    # check a spread of parameters of every tensor
+    picks = np.concatenate([rng.choice(lo.NPARAM - 1, 60, replace=False), [lo.NPARAM - 1]])
+    h = 1e-6
+    for i in picks:
+        e = np.zeros(lo.NPARAM)
+        e[i] = h
+        fd = (lo.loss_grad(p + e, x, t, emulate=False)[0] - lo.loss_grad(p - e, x, t, emulate=False)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+@pytest.mark.parametrize("dim,N,M", [(3, 24, 700), (2, 40, 900)])
+def test_neighbour_counts_match_kdtree(dim, N, M):
+    rng = np.random.default_rng(dim)
+    # clustered points so that counts are large and varied
+    t = np.concatenate([rng.normal(N / 2, 2.0, (M // 2, dim)), rng.uniform(-1, N, (M - M // 2, dim))])
+    t = t.astype(np.float32)
+    cell = np.clip(np.floor(t + np.float32(0.5)), 0, N - 1)
+    tree = cKDTree(cell)
+    for r in (1, 2, 4):
+        ref = np.array([len(q) - 1 for q in tree.query_ball_point(cell, r + 0.5, p=np.inf)])
+        assert np.array_equal(lo.neighbour_counts(t, N, r), ref)
+
+
+def test_angular_features_closed_forms():
+    N = 33                       # centre (N-1)/2 = 16, half width 16.5
+    c = 16.0
+    pts = np.array([[c + 8, c, c], [c, c + 8, c], [c, c, c + 8], [c, c - 8, c], [c - 8, c, c - 8]], np.float32)
+    f = lo.features(pts, N)
+    np.testing.assert_allclose(f[:, 0], [8 / 16.5, 8 / 16.5, 8 / 16.5, 8 / 16.5, np.sqrt(2) * 8 / 16.5], rtol=1e-6)
+    np.testing.assert_allclose(f[:, 1], [0.0, 0.5, 0.0, -0.5, 1.0], atol=1e-7)
+    np.testing.assert_allclose(f[:, 2], [0.5, 0.5, 0.0, 0.5, 0.75], atol=1e-7)
+    assert np.all(f[:, 3:] == 0)           # isolated points: no neighbours
