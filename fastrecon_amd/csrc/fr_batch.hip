@@ -80,19 +80,21 @@ extern "C" void fr_graph_destroy(fr_graph *gr)
     delete gr;
 }
 
-// Default candidates: unit boxes around the default (16-64 cells), each with
-// the automatic slice size and half / double of it.
+// Default candidates (FR_AUTOTUNE_DEFAULTS of them): unit boxes around the
+// default (16-64 cells) with LDS read-add-write accumulation, and the 32-cell
+// boxes with the gather kernel (cell-sorted batches, VGPR cells).
 static std::vector<fr_tune> default_candidates(int dim)
 {
     std::vector<fr_tune> c;
-    const int shapes3[][3] = {{4, 4, 2}, {4, 2, 2}, {4, 4, 4}, {8, 4, 2}, {2, 4, 4}, {8, 2, 2}};
-    const int shapes2[][3] = {{8, 4, 1}, {4, 4, 1}, {8, 8, 1}, {16, 4, 1}, {4, 8, 1}, {16, 2, 1}};
-    const int(*shapes)[3] = dim == 3 ? shapes3 : shapes2;
+    const int shapes3[][4] = {{4, 4, 2, 2}, {4, 2, 2, 2}, {4, 4, 4, 2}, {8, 4, 2, 2}, {4, 4, 2, 4}, {2, 4, 4, 4}};
+    const int shapes2[][4] = {{8, 4, 1, 2}, {4, 4, 1, 2}, {8, 8, 1, 2}, {16, 4, 1, 2}, {8, 4, 1, 4}, {4, 8, 1, 4}};
+    const int(*shapes)[4] = dim == 3 ? shapes3 : shapes2;
     for (int i = 0; i < 6; ++i) {
         fr_tune t{};
         t.tile_x = shapes[i][0];
         t.tile_y = shapes[i][1];
         t.tile_z = shapes[i][2];
+        t.accumulate = shapes[i][3];
         c.push_back(t);
     }
     return c;

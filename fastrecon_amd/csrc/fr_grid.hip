@@ -238,6 +238,163 @@ __global__ void __launch_bounds__(64) k_grid_items(const fr_item *__restrict__ i
 }
 
 
+// ------------------------------------------------------------------------
+// Gather kernel (FR_ACC_GATHER).  Same items and summation order, different
+// data flow: an item's records are processed in batches of GB; the batch's
+// sample rows, scaled by the DCF (d' = dcf * s, GridLut.cpp:109), are staged
+// in LDS as rows of 64 components, and its taps come sorted by cell.  The
+// unit's (at most 32) cells live in VGPRs, one per cell, indexed by the
+// unrolled cell loop, so a tap is one LDS read of the sample's row at the
+// wave's 64 components, one multiply and one add into a register: half the
+// LDS traffic of a read-add-write and no LDS write per tap.  Within a cell
+// the taps are in record (= ascending sample) order and the batches are in
+// order, so every cell sums in the reference order (exact mode: bitwise).
+// Padding taps read the batch's zero row with weight 0 (exact no-ops).
+constexpr int GB = FR_GATHER_BATCH;
+constexpr int GCELLS = FR_GATHER_CELLS;
+
+template <typename TS>
+struct gbatch {
+    float d[GB];                   // lane p: component p of record j's sample
+    float w;                       // lane j: DCF of record j
+    uint32_t k;                    // records
+};
+
+template <typename TS>
+__device__ __forceinline__ void gprep(gbatch<TS> &B, const fr_batch *__restrict__ batches, uint32_t bi, uint32_t nb,
+                                      const uint2 *__restrict__ H, const TS *__restrict__ data,
+                                      const float *__restrict__ dcf, int C, int lc, uint32_t vz, int lane)
+{
+    const bool ok = bi < nb;
+    const fr_batch Bd = batches[ok ? bi : 0];
+    const uint32_t k = ok ? Bd.nrec : 0u;
+    B.k = k;
+    const uint2 h = H[Bd.r0 + (lane < GB ? lane : 0) + vz];      // lane j: header of record j (H is padded)
+#pragma clang loop unroll(full)
+    for (int j = 0; j < GB; ++j) {
+        const uint32_t sidx = rl(h.x, (uint32_t)j < k ? j : 0);
+        B.d[j] = load_comp<TS>(data, sidx, C, lc);
+    }
+    B.w = dcf[h.x];
+}
+
+// rows of batch B -> LDS (row j at j * 64 floats)
+template <typename TS>
+__device__ __forceinline__ void gstage(const gbatch<TS> &B, float *rows, int lane)
+{
+#pragma clang loop unroll(full)
+    for (int j = 0; j < GB; ++j) {
+        const float w = __uint_as_float(rl(__float_as_uint(B.w), j));
+        if ((uint32_t)j < B.k) rows[j * 64 + lane] = __fmul_rn(w, B.d[j]);   // dcf * sample
+    }
+}
+
+__device__ __forceinline__ void gtaps(float (&acc)[GCELLS], const fr_batch &Bd, const uint2 *__restrict__ T,
+                                      const char *rowsb, uint32_t lane4)
+{
+    uint64_t t = Bd.toff;
+#pragma clang loop unroll(full)
+    for (int c = 0; c < GCELLS; ++c) {
+        const uint32_t n = (Bd.cnt[c >> 2] >> (8 * (c & 3))) & 255u;
+        const uint2 *tp = T + t;
+        for (uint32_t q = 0; q < n; q += 4) {
+            float v[4], x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint2 a = tp[q + u];
+                x[u] = *reinterpret_cast<const float *>(rowsb + a.x + lane4);
+                v[u] = __uint_as_float(a.y);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[c] = __fadd_rn(acc[c], __fmul_rn(v[u], x[u]));
+        }
+        t += n;
+    }
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(64) k_grid_gather(const fr_item *__restrict__ items, int64_t nitems,
+                                                    const fr_unit *__restrict__ units,
+                                                    const fr_batch *__restrict__ batches,
+                                                    const uint2 *__restrict__ H, const uint2 *__restrict__ T,
+                                                    const TS *__restrict__ data, const float *__restrict__ dcf, int C,
+                                                    int N, size_t cells, float2 *__restrict__ out,
+                                                    float2 *__restrict__ partial, int64_t nslots, int cells_max,
+                                                    int64_t nfront)
+{
+    // LDS: rows [GB + 1][64] (row GB stays zero); reused as [32][ACC_STRIDE] for the write-out
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x;
+    const int64_t bid = blockIdx.x;
+    int64_t i;
+    if (bid < nfront) {
+        i = bid;
+    } else {
+        const int64_t r = bid - nfront;
+        const int64_t g = r / 8;
+        const int64_t li = (g / XCD_RUN) * (8 * XCD_RUN) + (bid % 8) * XCD_RUN + g % XCD_RUN;
+        i = nfront + li;
+    }
+    if (i >= nitems) return;
+    const fr_item it = items[i];
+    const fr_unit U = units[it.unit];
+    const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
+    const int ncell = ux * uy * uz;
+    const int c0 = blockIdx.y * 32;
+    const int nc = min(32, C - c0);
+    const int lc = lane < 2 * nc ? 2 * c0 + lane : 2 * c0;
+    const uint32_t vz = opaque_zero();
+    const uint32_t lane4 = 4u * lane;
+    lds[GB * 64 + lane] = 0.f;
+    float acc[GCELLS];
+#pragma unroll
+    for (int c = 0; c < GCELLS; ++c) acc[c] = 0.f;
+
+    const uint32_t nb = (it.r1 - it.r0 + GB - 1) / GB;
+    const uint32_t b0 = it.t0;
+    const char *rowsb = reinterpret_cast<const char *>(lds);
+    if (nb > 0) {
+        gbatch<TS> A, B;
+        gprep(A, batches, b0, b0 + nb, H, data, dcf, C, lc, vz, lane);
+        for (uint32_t bi = b0;; bi += 2) {
+            // batch bi (in A): stage, prefetch bi + 1 into B, apply
+            gstage(A, lds, lane);
+            gprep(B, batches, bi + 1, b0 + nb, H, data, dcf, C, lc, vz, lane);
+            gtaps(acc, batches[bi], T, rowsb, lane4);
+            if (bi + 1 >= b0 + nb) break;
+            gstage(B, lds, lane);
+            gprep(A, batches, bi + 2, b0 + nb, H, data, dcf, C, lc, vz, lane);
+            gtaps(acc, batches[bi + 1], T, rowsb, lane4);
+            if (bi + 2 >= b0 + nb) break;
+        }
+    }
+
+    // accumulators -> LDS rows [cell][ACC_STRIDE], then coalesced writes
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < GCELLS; ++c) lds[c * ACC_STRIDE + lane] = acc[c];
+    __syncthreads();
+    const float *a = lds;
+    if (it.slot < 0) {
+        for (int q = lane; q < nc * ncell; q += 64) {
+            const int c = q / ncell;
+            const int cell = q - c * ncell;
+            const int lx = cell % ux;
+            const int ly = (cell / ux) % uy;
+            const int lz = cell / (ux * uy);
+            const size_t g = ((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx);
+            out[(size_t)(c0 + c) * cells + g] = make_float2(a[cell * ACC_STRIDE + 2 * c], a[cell * ACC_STRIDE + 2 * c + 1]);
+        }
+    } else {
+        float2 *pp = partial + ((size_t)blockIdx.y * nslots + it.slot) * (size_t)(32 * cells_max);
+        for (int q = lane; q < nc * ncell; q += 64) {
+            const int c = q / ncell;
+            const int cell = q - c * ncell;
+            pp[(size_t)c * cells_max + cell] = make_float2(a[cell * ACC_STRIDE + 2 * c], a[cell * ACC_STRIDE + 2 * c + 1]);
+        }
+    }
+}
+
 // Heavy units: sum their slices' partial boxes in slice order, write the grid.
 __global__ void k_combine(const fr_heavy *__restrict__ heavy, const fr_unit *__restrict__ units,
                           const float2 *__restrict__ partial, int64_t nslots, int cells_max, int C, int N, size_t cells,
@@ -295,11 +452,18 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
     const int64_t light = p->nitems - p->nitems_front;
     const int64_t span = 8 * XCD_RUN;
     dim3 grid((unsigned)(p->nitems_front + span * ((light + span - 1) / span)), chunks);
-    auto kern = k_grid_items<ACC == FR_ACC_LDS_ATOMIC ? 1 : 0, TS>;
-    FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, grid, dim3(64), lds, ctx->stream, p->items, p->nitems, p->units, p->hdrs, p->taps,
-                       (const TS *)s->data, s->dcf, s->C, stride, acc_floats, (int)p->N, cells, out,
-                       (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
+    if constexpr (ACC == FR_ACC_GATHER) {
+        const size_t glds = sizeof(float) * (size_t)(GB + 1) * 64;
+        hipLaunchKernelGGL(k_grid_gather<TS>, grid, dim3(64), glds, ctx->stream, p->items, p->nitems, p->units,
+                           p->batches, p->hdrs, p->taps, (const TS *)s->data, s->dcf, s->C, (int)p->N, cells, out,
+                           (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
+    } else {
+        auto kern = k_grid_items<ACC == FR_ACC_LDS_ATOMIC ? 1 : 0, TS>;
+        FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, grid, dim3(64), lds, ctx->stream, p->items, p->nitems, p->units, p->hdrs, p->taps,
+                           (const TS *)s->data, s->dcf, s->C, stride, acc_floats, (int)p->N, cells, out,
+                           (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
+    }
     FR_HIP(ctx, hipGetLastError());
     if (p->nheavy > 0) {
         hipLaunchKernelGGL(k_combine, dim3((unsigned)p->nheavy, chunks), dim3(256), 0, ctx->stream, p->heavy,
@@ -315,6 +479,8 @@ fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int)
 {
     const bool c64 = s->type == FR_SAMPLES_C64;
     switch (p->accumulate) {
+    case FR_ACC_GATHER:
+        return c64 ? launch<FR_ACC_GATHER, float2>(p, s, out) : launch<FR_ACC_GATHER, __half2>(p, s, out);
     case FR_ACC_LDS_ATOMIC:
         return c64 ? launch<FR_ACC_LDS_ATOMIC, float2>(p, s, out) : launch<FR_ACC_LDS_ATOMIC, __half2>(p, s, out);
     default:

@@ -58,6 +58,18 @@ struct fr_grids {
 #define FR_ACC_LDS_ATOMIC 1
 #define FR_ACC_LDS_RMW 2
 #define FR_ACC_REGS 3       // refused: see fr_plan_create
+#define FR_ACC_GATHER 4     // cell-sorted batches, sample rows in LDS, VGPR accumulators
+#define FR_GATHER_BATCH 32  // records per gather batch
+#define FR_GATHER_CELLS 32  // unit cells held in VGPRs by the gather kernel
+
+// One batch of a gather-mode item: records [r0, r0 + nrec) and their taps
+// T[toff ..) sorted by cell; cnt byte k = tap count of cell k (a multiple
+// of 4; padding taps read the batch's zero row with weight 0).
+struct fr_batch {
+    uint64_t toff;
+    uint32_t r0, nrec;
+    uint32_t cnt[FR_GATHER_CELLS / 4];
+};
 
 // A work unit is a box of grid cells owned by one wave.  Its records (one
 // per sample that has taps in the box, ascending sample order) are H[hdr ..
@@ -125,6 +137,8 @@ struct fr_plan {
     int64_t nheavy = 0, nslots = 0;
     float *partial = nullptr;          // scratch for heavy-unit slices
     size_t partial_bytes = 0;
+    fr_batch *batches = nullptr;       // gather mode: item i owns batches [items[i].t0, ..)
+    int64_t nbatches = 0;
     double build_ms = 0.0;
 };
 

@@ -257,6 +257,84 @@ __global__ void k_make_heavy(int64_t nunits, const uint32_t *__restrict__ hflag,
     for (uint32_t k = 0; k < nsl[u]; ++k) items[first[u] + k].slot = (int32_t)(slot0 + k);
 }
 
+// ---- gather mode: batches of FR_GATHER_BATCH records with cell-sorted taps
+__global__ void k_item_nbatch(const fr_item *__restrict__ items, int64_t nitems, uint32_t *__restrict__ nb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    nb[i] = (items[i].r1 - items[i].r0 + FR_GATHER_BATCH - 1) / FR_GATHER_BATCH;
+}
+
+__global__ void k_make_batches(fr_item *__restrict__ items, int64_t nitems, const uint32_t *__restrict__ nb,
+                               const uint32_t *__restrict__ scan, fr_batch *__restrict__ batches)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    const uint32_t b0 = scan[i] - nb[i];            // inclusive scan
+    const fr_item it = items[i];
+    for (uint32_t b = 0; b < nb[i]; ++b) {
+        fr_batch B{};
+        B.r0 = it.r0 + b * FR_GATHER_BATCH;
+        B.nrec = min((uint32_t)FR_GATHER_BATCH, it.r1 - B.r0);
+        batches[b0 + b] = B;
+    }
+    items[i].t0 = b0;
+}
+
+// One wave per batch; lane c < FR_GATHER_CELLS counts the batch's taps of
+// cell c (at most one per record) and pads the count to a multiple of 4.
+__global__ void k_batch_count(fr_batch *__restrict__ batches, int64_t nbat, const uint2 *__restrict__ H,
+                              const uint64_t *__restrict__ toff, const uint2 *__restrict__ T,
+                              uint64_t *__restrict__ bsize)
+{
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= nbat) return;
+    const fr_batch B = batches[w];
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < B.nrec; ++j) {
+        const uint64_t r = B.r0 + j;
+        const uint64_t t0 = toff[r];
+        const uint32_t np = H[r].y;
+        for (uint32_t t = 0; t < np; ++t) n += T[t0 + t].x == (uint32_t)lane;
+    }
+    const uint32_t p = lane < FR_GATHER_CELLS ? (n + 3u) & ~3u : 0u;
+    if (lane < FR_GATHER_CELLS) reinterpret_cast<uint8_t *>(batches[w].cnt)[lane] = (uint8_t)p;
+    uint32_t sum = p;
+    for (int m = 32; m >= 1; m >>= 1) sum += __shfl_xor(sum, m);
+    if (lane == 0) bsize[w] = sum;
+}
+
+__global__ void k_batch_write(fr_batch *__restrict__ batches, int64_t nbat, const uint2 *__restrict__ H,
+                              const uint64_t *__restrict__ toff, const uint2 *__restrict__ T,
+                              const uint64_t *__restrict__ bstart, uint2 *__restrict__ T2)
+{
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= nbat) return;
+    const fr_batch B = batches[w];
+    const uint64_t base = bstart[w] - [&] {                    // inclusive scan -> exclusive start
+        uint32_t s = 0;
+        for (int k = 0; k < FR_GATHER_CELLS; ++k) s += reinterpret_cast<const uint8_t *>(B.cnt)[k];
+        return (uint64_t)s;
+    }();
+    if (lane == 0) batches[w].toff = base;
+    if (lane >= FR_GATHER_CELLS) return;
+    uint64_t o = base;
+    for (int k = 0; k < lane; ++k) o += reinterpret_cast<const uint8_t *>(B.cnt)[k];
+    const uint64_t end = o + reinterpret_cast<const uint8_t *>(B.cnt)[lane];
+    for (uint32_t j = 0; j < B.nrec; ++j) {
+        const uint64_t r = B.r0 + j;
+        const uint64_t t0 = toff[r];
+        const uint32_t np = H[r].y;
+        for (uint32_t t = 0; t < np; ++t) {
+            const uint2 a = T[t0 + t];
+            if (a.x == (uint32_t)lane) T2[o++] = make_uint2(j * 256u, a.y);
+        }
+    }
+    for (; o < end; ++o) T2[o] = make_uint2(FR_GATHER_BATCH * 256u, 0u);
+}
+
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 template <typename T>
@@ -362,8 +440,11 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     (void)stride;
     p->geom = geom;
     p->accumulate = tune ? tune->accumulate : 0;
-    FR_CHECK(ctx, p->accumulate >= 0 && p->accumulate <= FR_ACC_REGS, FR_ERR_INVALID, "unknown accumulate mode %d",
+    FR_CHECK(ctx, p->accumulate >= 0 && p->accumulate <= FR_ACC_GATHER, FR_ERR_INVALID, "unknown accumulate mode %d",
              p->accumulate);
+    FR_CHECK(ctx, p->accumulate != FR_ACC_GATHER || geom.cells_max <= FR_GATHER_CELLS, FR_ERR_INVALID,
+             "gather accumulation needs a unit box of at most %d cells (have %dx%dx%d)", FR_GATHER_CELLS, geom.bx,
+             geom.by, geom.bz);
     // VGPR accumulation (GPR-index addressed adds) gave wrong sums and a
     // memory fault on gfx950 whose cause is not established: it is refused.
     FR_CHECK(ctx, p->accumulate != FR_ACC_REGS, FR_ERR_UNSUPPORTED,
@@ -516,8 +597,11 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     FR_HIP(ctx, hipMemsetAsync(p->hdrs, 0, sizeof(uint2) * (R + 64), st));
     // tap cell field: the byte offset of the cell's accumulator row; padding
     // taps go to the spare row cells_max with weight 0
-    const uint32_t tap_scale = 4u * FR_ACC_STRIDE;
-    const uint32_t pad_x = (uint32_t)geom.cells_max * 4u * FR_ACC_STRIDE;
+    // (gather mode: the cell index, padding marked 0xffff; the batch build
+    // below re-sorts them)
+    const bool gather = p->accumulate == FR_ACC_GATHER;
+    const uint32_t tap_scale = gather ? 1u : 4u * FR_ACC_STRIDE;
+    const uint32_t pad_x = gather ? 0xffffu : (uint32_t)geom.cells_max * 4u * FR_ACC_STRIDE;
     if (R) {
         hipLaunchKernelGGL(k_unit_ranges, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, (int64_t)R,
                            p->units);
@@ -593,6 +677,54 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     }
     p->nrecords = (int64_t)R;
     p->ntaps = (int64_t)TT;
+
+    // 7. gather mode: split every item into batches of FR_GATHER_BATCH
+    // records and re-sort each batch's taps by cell (stable: record order)
+    if (gather && p->nitems > 0) {
+        const int64_t ni = p->nitems;
+        dbuf<uint32_t> nb, nbs;
+        if ((e = fr_alloc(ctx, (void **)&nb.p, 4 * ni)) || (e = fr_alloc(ctx, (void **)&nbs.p, 4 * ni))) return e;
+        hipLaunchKernelGGL(k_item_nbatch, dim3(nblk(ni, 256)), dim3(256), 0, st, p->items, ni, nb.p);
+        size_t sb = 0;
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, sb, nb.p, nbs.p, (int)ni, st));
+        dbuf<char> stmp2;
+        if ((e = fr_alloc(ctx, (void **)&stmp2.p, sb + 16))) return e;
+        FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(stmp2.p, sb, nb.p, nbs.p, (int)ni, st));
+        uint32_t nbat = 0;
+        FR_HIP(ctx, hipMemcpyAsync(&nbat, nbs.p + ni - 1, 4, hipMemcpyDeviceToHost, st));
+        FR_HIP(ctx, hipStreamSynchronize(st));
+        p->nbatches = nbat;
+        if ((e = fr_alloc(ctx, (void **)&p->batches, sizeof(fr_batch) * std::max<uint32_t>(1, nbat)))) return e;
+        hipLaunchKernelGGL(k_make_batches, dim3(nblk(ni, 256)), dim3(256), 0, st, p->items, ni, nb.p, nbs.p,
+                           p->batches);
+        dbuf<uint64_t> bsize, bscan;
+        if ((e = fr_alloc(ctx, (void **)&bsize.p, 8 * (uint64_t)std::max<uint32_t>(1, nbat))) ||
+            (e = fr_alloc(ctx, (void **)&bscan.p, 8 * (uint64_t)std::max<uint32_t>(1, nbat))))
+            return e;
+        uint64_t T2n = 0;
+        if (nbat) {
+            hipLaunchKernelGGL(k_batch_count, dim3(nblk((int64_t)nbat * 64, 256)), dim3(256), 0, st, p->batches,
+                               (int64_t)nbat, p->hdrs, toff.p, p->taps, bsize.p);
+            size_t sb2 = 0;
+            FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, sb2, bsize.p, bscan.p, (int)nbat, st));
+            dbuf<char> stmp3;
+            if ((e = fr_alloc(ctx, (void **)&stmp3.p, sb2 + 16))) return e;
+            FR_HIP(ctx, hipcub::DeviceScan::InclusiveSum(stmp3.p, sb2, bsize.p, bscan.p, (int)nbat, st));
+            FR_HIP(ctx, hipMemcpyAsync(&T2n, bscan.p + nbat - 1, 8, hipMemcpyDeviceToHost, st));
+            FR_HIP(ctx, hipStreamSynchronize(st));
+        }
+        uint2 *T2 = nullptr;
+        if ((e = fr_alloc(ctx, (void **)&T2, sizeof(uint2) * (T2n + 64)))) return e;
+        FR_HIP(ctx, hipMemsetAsync(T2, 0, sizeof(uint2) * (T2n + 64), st));
+        if (nbat)
+            hipLaunchKernelGGL(k_batch_write, dim3(nblk((int64_t)nbat * 64, 256)), dim3(256), 0, st, p->batches,
+                               (int64_t)nbat, p->hdrs, toff.p, p->taps, bscan.p, T2);
+        FR_HIP(ctx, hipGetLastError());
+        FR_HIP(ctx, hipStreamSynchronize(st));
+        hipFree(p->taps);
+        p->taps = T2;
+        p->ntaps = (int64_t)T2n;
+    }
     p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return FR_OK;
 }
@@ -631,7 +763,7 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
     if (!p) return FR_OK;
     hipStreamSynchronize(p->ctx->stream);
     hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps);
-    hipFree(p->items); hipFree(p->heavy); hipFree(p->partial);
+    hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches);
     delete p;
     return FR_OK;
 }

@@ -59,7 +59,10 @@ typedef struct {
     int accumulate;              /* where a unit's cells accumulate:
                                     0: automatic (LDS read-add-write),
                                     1: LDS float atomic add, 2: LDS read-add-write,
-                                    3: VGPRs -- refused (FR_ERR_UNSUPPORTED) */
+                                    3: VGPRs -- refused (FR_ERR_UNSUPPORTED),
+                                    4: gather -- per-batch cell-sorted taps, the
+                                       batch's sample rows in LDS, one VGPR per
+                                       cell (unit box of at most 32 cells) */
     int split_taps;              /* heavier units are split (0: automatic)      */
     int exact_order;             /* 1: split units by space only, so every cell
                                     sums in ascending sample order (bit-identical
@@ -226,10 +229,11 @@ fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *g, fr_fft *
                           unsigned rcz, int apply_shift, const float *deapod_dev, float *out_dev, fr_graph **out);
 fr_status fr_graph_launch(fr_graph *gr);
 void fr_graph_destroy(fr_graph *gr);
-/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: unit
- * shapes around the default) on these samples and returns the fastest in
- * *best (and its time); cand_ms (optional, ncand or 6 entries) gets each
- * candidate's time, -1 for candidates the plan refuses.  The LUT length is
+/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: six
+ * defaults -- unit shapes around the default with LDS read-add-write, and
+ * 32-cell shapes with the gather kernel) on these samples and returns the
+ * fastest in *best (and its time); cand_ms (optional, ncand or 6 entries)
+ * gets each candidate's time, -1 for candidates the plan refuses.  The LUT length is
  * not tuned: tap weights are resolved when the plan is built, so L does not
  * change the pass time, and L = 512 keeps reference parity. */
 fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, const float *lut, int L,

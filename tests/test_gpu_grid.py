@@ -63,22 +63,23 @@ def check_exact(got, ref, label):
     np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
 
 
-def test_radial2d_full_config_bitwise(ctx, orc):
+@pytest.mark.parametrize("acc", [0, 4])
+def test_radial2d_full_config_bitwise(ctx, orc, acc):
     # the CPU-runnable config: 2D golden-angle, 402 x 512, 8 coils, 512^2 grid
     tn, dcf, data, N, lut = case(orc, "radial2d", 402, 512, 8, seed=1)
-    print("radial2d: M=%d N=%d" % (tn.shape[0], N), flush=True)
-    got, info = run_grid(ctx, tn, dcf, data, N, lut)
+    print("radial2d: M=%d N=%d accumulate=%d" % (tn.shape[0], N, acc), flush=True)
+    got, info = run_grid(ctx, tn, dcf, data, N, lut, tune=dict(EXACT, accumulate=acc))
     ref = orc.grid_f32(tn, dcf, data, N, 4.0, lut, nthreads=8)
     check_exact(got, ref, "radial2d")
     ref64 = orc.grid_f64(tn, dcf, data, N, 4.0, lut, nthreads=8)
     print(f"  vs FP64 oracle: gpu {rel_l2(got, ref64):.3e}  cpu-fp32 {rel_l2(ref, ref64):.3e}", flush=True)
 
 
-@pytest.mark.parametrize("coils", [1, 4, 32, 40])
-def test_kooshball3d_small_bitwise(ctx, orc, coils):
+@pytest.mark.parametrize("coils,acc", [(1, 0), (4, 0), (32, 0), (40, 0), (1, 4), (32, 4), (40, 4)])
+def test_kooshball3d_small_bitwise(ctx, orc, coils, acc):
     tn, dcf, data, N, lut = case(orc, "kooshball3d", 600, 64, coils, seed=2, image=32)
-    print("kooshball small: M=%d N=%d C=%d" % (tn.shape[0], N, coils), flush=True)
-    got, info = run_grid(ctx, tn, dcf, data, N, lut)
+    print("kooshball small: M=%d N=%d C=%d accumulate=%d" % (tn.shape[0], N, coils, acc), flush=True)
+    got, info = run_grid(ctx, tn, dcf, data, N, lut, tune=dict(EXACT, accumulate=acc))
     ref = orc.grid_f32(tn, dcf, data, N, 4.0, lut, nthreads=8)
     check_exact(got, ref, f"koosh C={coils}")
 
@@ -92,7 +93,10 @@ REGS_OFF = pytest.mark.skip(reason="VGPR accumulation (accumulate=3) gave wrong 
                                   dict(tile_x=8, tile_y=4, tile_z=2), dict(tile_x=4, tile_y=4, tile_z=4),
                                   pytest.param(dict(tile_x=2, tile_y=4, tile_z=4, accumulate=3), marks=REGS_OFF),
                                   dict(tile_x=8, tile_y=2, tile_z=2), dict(tile_x=3, tile_y=3, tile_z=3),
-                                  dict(split_taps=64), dict(split_taps=1), dict(split_taps=1, accumulate=2)])
+                                  dict(split_taps=64), dict(split_taps=1), dict(split_taps=1, accumulate=2),
+                                  dict(accumulate=4), dict(accumulate=4, tile_x=2, tile_y=4, tile_z=4),
+                                  dict(accumulate=4, split_taps=64), dict(accumulate=4, tile_x=3, tile_y=3,
+                                                                          tile_z=3)])
 def test_tile_shape_and_accumulate_mode_invariance(ctx, orc, tune):
     tn, dcf, data, N, lut = case(orc, "kooshball3d", 300, 48, 6, seed=5, image=24)
     got, info = run_grid(ctx, tn, dcf, data, N, lut, tune=dict(tune, exact_order=1))
@@ -135,9 +139,10 @@ def test_single_sample_and_determinism(ctx, orc):
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
 
 
-def test_fp16_samples_against_dequantised_oracle(ctx, orc):
+@pytest.mark.parametrize("acc", [0, 4])
+def test_fp16_samples_against_dequantised_oracle(ctx, orc, acc):
     tn, dcf, data, N, lut = case(orc, "kooshball3d_fp16", 500, 96, 16, seed=3, image=48)
-    got, _ = run_grid(ctx, tn, dcf, data, N, lut, sample_type=hip.FR_SAMPLES_C16)
+    got, _ = run_grid(ctx, tn, dcf, data, N, lut, tune=dict(EXACT, accumulate=acc), sample_type=hip.FR_SAMPLES_C16)
     deq = (data.real.astype(np.float16).astype(np.float32)
            + 1j * data.imag.astype(np.float16).astype(np.float32)).astype(np.complex64)
     ref = orc.grid_f32(tn, dcf, deq, N, 4.0, lut, nthreads=8)
@@ -166,6 +171,8 @@ def test_update_acquisition(ctx, orc):
 
 @pytest.mark.parametrize("coils,tune", [(32, {}), (8, {}), (40, {}), (32, dict(split_taps=300)),
                                         (32, dict(split_taps=300, accumulate=2)),
+                                        (32, dict(accumulate=4)), (8, dict(accumulate=4)),
+                                        (40, dict(accumulate=4, split_taps=300)),
                                         (32, dict(split_taps=2000, tile_x=4, tile_y=4, tile_z=4))])
 def test_sliced_mode_parity_and_determinism(ctx, orc, coils, tune):
     """Default mode: heavy units are cut by sample range; the slices are
