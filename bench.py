@@ -38,10 +38,12 @@ def parse():
     ap.add_argument("--readout", type=int, default=256)
     ap.add_argument("--coils", type=int, default=32)
     ap.add_argument("--image", type=int, default=128)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work (seconds)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recon", action="store_true")
     ap.add_argument("--no-ldcf", action="store_true", help="skip the learned-vs-iterative DCF leg")
+    ap.add_argument("--no-frames", action="store_true", help="skip the batched 2D spiral frames leg")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-sample 384^3 leg")
     ap.add_argument("--accumulate", type=int, default=0)
     ap.add_argument("--exact", action="store_true", help="exact-order plan for the timed steps")
     ap.add_argument("--unit", type=int, nargs=3, default=None, metavar=("X", "Y", "Z"),
@@ -85,6 +87,100 @@ def read_traffic(path, workload):
         return None, None
 
 
+def time_stream(torch, ctx, fn, steps):
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    ctx.sync()
+    return e0.elapsed_time(e1) / steps
+
+
+def device_data(torch, ctx, samples, M, C, seed):
+    """Seeded complex Gaussian coil data generated on the device, sample-major."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    d = torch.randn(M * C * 2, generator=gen, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+    samples.set_data_device(d.data_ptr(), 1)
+    ctx.sync()
+    del d
+
+
+def bench_frames(ctx, torch, hip, syn, steps):
+    """64 uneven 2D spiral frames (8-48 interleaves x 4096 samples), 12 coils,
+    192^2 image (384^2 grid): one batched plan, every frame gridded in one
+    launch, the whole series (grid + batched FFT + per-frame shift/crop/SOS)
+    replayed as one HIP graph; the autotuner's pick is timed too."""
+    C, image = 12, 192
+    N = syn.grid_size(image, 2.0)
+    frames, counts = syn.spiral_frames(64, 4096)
+    tns = [hip.normalize_traj(k, N)[0] for k, _ in frames]
+    sizes = np.array([t.shape[0] for t in tns], np.int64)
+    M = int(sizes.sum())
+    s = hip.Samples(ctx, 2, M, C, frame_samples=sizes)
+    s.set_traj(np.concatenate(tns))
+    s.set_dcf(np.concatenate([d for _, d in frames]))
+    device_data(torch, ctx, s, M, C, 4)
+    lut, _ = hip.kb_lut(4.0, 2.0, 512)
+    F = len(tns)
+    res = {"workload": f"64 uneven 2D spiral frames ({int(counts.min())}-{int(counts.max())} interleaves x 4096), "
+                       f"{C} coils, {image}^2 image, og 2 -> {N}^2 grid", "samples_total": M}
+    t0 = time.perf_counter()
+    best, best_ms, cand = hip.autotune(ctx, s, N, 4.0, lut, reps=3)
+    res["autotune_s"] = time.perf_counter() - t0
+    res["autotune_candidates_ms"] = [round(float(x), 4) for x in cand]
+    res["autotune_best"] = {k: v for k, v in best.items() if v}
+    for label, tune in (("default", None), ("tuned", best)):
+        t0 = time.perf_counter()
+        p = hip.Plan(ctx, s, N, 4.0, lut, tune)
+        plan_ms = (time.perf_counter() - t0) * 1e3
+        g = hip.Grids(ctx, 2, N, C, frames=F)
+        fft = hip.FFT(ctx, 2, N, C * F, +1)
+        img = hip.DeviceBuffer(ctx, 4 * image * image * F)
+        graph = hip.Graph(p, s, g, fft, (image, image, 1), True, None, img)
+        grid_ms = time_stream(torch, ctx, lambda: hip.grid_execute(p, s, g), steps)
+        series_ms = time_stream(torch, ctx, graph.launch, steps)
+        res[label] = {"plan_ms": plan_ms, "grid_ms": grid_ms, "series_ms_graph": series_ms,
+                      "gridded_coil_samples_per_s": C * M / (grid_ms * 1e-3)}
+        for h in (graph, img, fft, g, p):
+            h.close()
+    s.close()
+    return res
+
+
+def bench_fp16(ctx, torch, hip, syn, steps):
+    """3D kooshball 100,000 spokes x 384, 16 coils, samples stored as FP16
+    pairs (FP32 accumulation), 256^3 image, og 1.5 -> 384^3 grid."""
+    C, image, og = 16, 256, 1.5
+    N = syn.grid_size(image, og)
+    traj, dcf = syn.kooshball3d(100000, 384)
+    M = traj.shape[0]
+    tn, _ = hip.normalize_traj(traj, N)
+    del traj
+    s = hip.Samples(ctx, 3, M, C, hip.FR_SAMPLES_C16)
+    s.set_traj(tn)
+    s.set_dcf(dcf)
+    device_data(torch, ctx, s, M, C, 3)
+    lut, _ = hip.kb_lut(4.0, og, 512)
+    t0 = time.perf_counter()
+    p = hip.Plan(ctx, s, N, 4.0, lut)
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    g = hip.Grids(ctx, 3, N, C)
+    hip.grid_execute(p, s, g)
+    ms = time_stream(torch, ctx, lambda: hip.grid_execute(p, s, g), steps)
+    b = C * M * 4 + M * 16 + C * N ** 3 * 8
+    res = {"workload": f"kooshball3d 100000 spokes x 384, {C} coils FP16, {image}^3 image, og 1.5 -> {N}^3 grid",
+           "samples_per_coil": M, "plan_ms": plan_ms, "grid_ms": ms,
+           "gridded_coil_samples_per_s": C * M / (ms * 1e-3), "algorithmic_bytes": b,
+           "achieved_gbs": b / (ms * 1e-3) / 1e9, "frac_of_8tbs": b / (ms * 1e-3) / 8e12}
+    for h in (g, p, s):
+        h.close()
+    return res
+
+
 def cpu_baseline(traj_n, dcf, N, coils, lut, target_s):
     """Oracle FP32 restatement of GridLut::griddingChannel on a bounded
     slice (the first spokes of the same trajectory), OpenMP over coils with
@@ -104,19 +200,19 @@ def cpu_baseline(traj_n, dcf, N, coils, lut, target_s):
         return (rng.standard_normal((coils, n), np.float32)
                 + 1j * rng.standard_normal((coils, n), np.float32)).astype(np.complex64)
 
-    tr = traj_n[:m]
-    d = data(m)
-    t0 = time.perf_counter()
-    oracle.grid_f32(tr, dcf[:m], d, N, 4.0, lut, nthreads=threads)
-    dt = time.perf_counter() - t0
-    # scale the slice to the target time
-    scale = max(1, int(target_s / max(dt, 1e-3)))
-    m2 = min(m * scale, traj_n.shape[0])
-    tr = traj_n[:m2]
-    d = data(m2)
-    t0 = time.perf_counter()
-    oracle.grid_f32(tr, dcf[:m2], d, N, 4.0, lut, nthreads=threads)
-    dt = time.perf_counter() - t0
+    # grow the slice until one timed run takes at least target_s (or the
+    # whole trajectory is used)
+    M = traj_n.shape[0]
+    while True:
+        tr = traj_n[:m]
+        d = data(m)
+        t0 = time.perf_counter()
+        oracle.grid_f32(tr, dcf[:m], d, N, 4.0, lut, nthreads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= target_s or m >= M:
+            break
+        m = min(M, max(2 * m, int(m * target_s / max(dt, 1e-3) * 1.1)) // readout * readout)
+    m2 = m
     return {"value": coils * m2 / dt, "unit": "coil-samples/s", "cores": threads, "kind": "port",
             "sample": f"first {m2 // readout} spokes x {readout} samples x {coils} coils of the same workload "
                       f"on the full {N}^3 grid, {dt:.2f} s, oracle FP32 restatement of GridLut.cpp:80-147 "
@@ -239,6 +335,14 @@ def main():
         fft.close()
         img.close()
 
+    if not a.no_frames:
+        grids.close()
+        torch.cuda.empty_cache()
+        out["frames_spiral2d"] = bench_frames(ctx, torch, hip, syn, a.steps)
+        grids = hip.Grids(ctx, 3, N, C)
+    if not a.no_fp16:
+        out["fp16_kooshball3d"] = bench_fp16(ctx, torch, hip, syn, max(3, a.steps // 4))
+
     if not a.no_ldcf:
         # learned vs iterative DCF on this workload's geometry (the network's
         # checkpoint was trained on the device by fastrecon_amd.ldcf_train)
@@ -259,6 +363,8 @@ def main():
 
         r = ldcf.evaluate(ctx, mlp, traj, fill, N, (a.image, a.image, a.image), ramp=dcf, reps=2, coils=C)
         out["ldcf"] = dict(r, iterations=10, checkpoint_rows=meta.get("rows"), checkpoint_loss=meta["losses"][-1][1])
+        out["ldcf"]["phantom_single_coil"] = ldcf.evaluate_phantom(ctx, mlp, traj, N, (a.image,) * 3, a.image,
+                                                                   ramp=dcf)
         mlp.close()
 
     if not a.no_cpu:

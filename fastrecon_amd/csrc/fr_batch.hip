@@ -80,7 +80,7 @@ extern "C" void fr_graph_destroy(fr_graph *gr)
     delete gr;
 }
 
-// Default candidates (FR_AUTOTUNE_DEFAULTS of them): unit boxes around the
+// Six default candidates: unit boxes around the
 // default (16-64 cells) with LDS read-add-write accumulation, and the 32-cell
 // boxes with the gather kernel (cell-sorted batches, VGPR cells).
 static std::vector<fr_tune> default_candidates(int dim)

@@ -5,7 +5,9 @@
 //
 // Features (one row of 8 floats per sample, the last two zero):
 //   u = (t - (N-1)/2) / (N/2) for the normalised point t,
-//   f0 = |u|, f1 = atan2(u_y, u_x)/pi, f2 = acos(u_z/|u|)/pi (0.5 in 2D),
+//   f0 = log1p(|u| N/2)/5 (the radius |k| in grid cells, log-compressed so
+//   that the first cells around the k-space centre are resolved),
+//   f1 = atan2(u_y, u_x)/pi, f2 = acos(u_z/|u|)/pi (0.5 in 2D),
 //   f3..f5 = log1p(n_r)/8 with n_r the number of other samples whose nearest
 //   cell lies within Chebyshev distance r = 1, 2, 4 of this sample's cell.
 // The counts come from a cell histogram and its summed-area table, so each
@@ -137,7 +139,7 @@ __global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, i
     const float r = sqrtf(r2);
     const float inv_pi = 0.318309886183790671f;
     float f[FSTRIDE];
-    f[0] = r;
+    f[0] = log1pf(r * hN) * 0.2f;             // radius in cells, log-compressed
     f[1] = atan2f(u[1], u[0]) * inv_pi;
     f[2] = (dim == 3 && r > 0.f) ? acosf(fminf(fmaxf(u[2] / r, -1.f), 1.f)) * inv_pi : 0.5f;
     const int rr[3] = {1, 2, 4};

@@ -90,6 +90,24 @@ def recon_image(ctx, plan, samples, N, rc, fft=None, grids=None):
 
 
 # ---------------------------------------------------------------- dataset
+def stratified_rows(radius, count, seed, offset, bins=32):
+    """`count` row indices drawn (seeded, with replacement) equally from
+    `bins` radius bins: the few samples near the k-space centre, which carry
+    most of an object's energy, are as well represented as the many outer
+    ones."""
+    r = np.asarray(radius, np.float64)
+    edges = np.linspace(0.0, r.max() + 1e-9, bins + 1)
+    b = np.clip(np.searchsorted(edges, r, side="right") - 1, 0, bins - 1)
+    order = np.argsort(b, kind="stable")
+    starts = np.searchsorted(b[order], np.arange(bins))
+    ends = np.searchsorted(b[order], np.arange(bins), side="right")
+    filled = [k for k in range(bins) if ends[k] > starts[k]]
+    u = syn.uniform(np.arange(count, dtype=np.uint64) + np.uint64(offset), seed)
+    which = np.array(filled)[np.arange(count) % len(filled)]
+    lo, hi = starts[which], ends[which]
+    return order[lo + np.minimum((u * (hi - lo)).astype(np.int64), hi - lo - 1)]
+
+
 def build_dataset(ctx, count=200, per_traj=50000, scale=1.0, seed=7, iters=10, log=print):
     """Features and Pipe-Menon targets of `count` seeded training trajectories,
     `per_traj` rows drawn (seeded) from each."""
@@ -98,8 +116,7 @@ def build_dataset(ctx, count=200, per_traj=50000, scale=1.0, seed=7, iters=10, l
     for n, (name, k) in enumerate(syn.ldcf_training_trajectories(count, seed, scale)):
         tn, _ = hip.normalize_traj(k, N_GRID)
         w, feat = pipe_menon_dcf(ctx, tn, N_GRID, iters)
-        rows = syn.uniform(np.arange(per_traj, dtype=np.uint64) + np.uint64(per_traj * n), seed + 1)
-        pick = np.minimum((rows * len(w)).astype(np.int64), len(w) - 1)
+        pick = stratified_rows(feat[:, 0], per_traj, seed + 1, per_traj * n)
         X.append(feat[pick])
         W.append(w[pick])
         if log and (n % 20 == 0 or n == count - 1):
@@ -230,7 +247,12 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     im_ml = img.to_host(np.float32)
     w_ml = s.get_dcf()
     t_rc = timed(recon)
+    # one Pipe-Menon iteration (1 / kernel density) for reference
+    hip.pipe_menon(p, s, 1, True)
+    recon()
+    im_pm1 = img.to_host(np.float32)
     out = dict(ms_iterative=t_it, ms_learned=t_ml, ms_recon_only=t_rc, speedup=t_it / t_ml,
+               nrmse_pm1=nrmse(im_pm1, im_it), nrmse_pm1_scaled=nrmse_scaled(im_pm1, im_it),
                nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
                dcf_rel_rms=float(np.sqrt(np.mean((w_ml / np.maximum(w_it, 1e-30) - 1.0) ** 2))))
     if ramp is not None:
@@ -253,4 +275,61 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
         h.close()
     if log:
         log(out)
+    return out
+
+
+# Axis-aligned ellipsoids (value, semi-axes a b c, centre x y z) in units of
+# the image half-width, after the 3D Shepp-Logan layout of Kak & Slaney.
+PHANTOM = [
+    (1.0, 0.69, 0.92, 0.81, 0.0, 0.0, 0.0),
+    (-0.8, 0.6624, 0.874, 0.78, 0.0, -0.0184, 0.0),
+    (-0.2, 0.11, 0.31, 0.22, 0.22, 0.0, 0.0),
+    (-0.2, 0.16, 0.41, 0.28, -0.22, 0.0, 0.0),
+    (0.1, 0.21, 0.25, 0.41, 0.0, 0.35, -0.15),
+    (0.1, 0.046, 0.046, 0.05, 0.0, 0.1, 0.25),
+    (0.1, 0.046, 0.046, 0.05, 0.0, -0.1, 0.25),
+    (0.1, 0.046, 0.023, 0.05, -0.08, -0.605, 0.0),
+    (0.1, 0.023, 0.023, 0.02, 0.0, -0.606, 0.0),
+    (0.1, 0.023, 0.046, 0.02, 0.06, -0.605, 0.0),
+]
+
+
+def phantom_kspace(k, image):
+    """Exact k-space of the ellipsoid phantom at points k (cycles per pixel,
+    M x 3): the Fourier transform of an ellipsoid of semi-axes (a, b, c) is
+    abc * 4 pi (sin q - q cos q) / q^3 with q = 2 pi |(a kx, b ky, c kz)|,
+    times the phase of its centre."""
+    k = np.asarray(k, np.float64)
+    half = image / 2.0
+    out = np.zeros(k.shape[0], np.complex128)
+    for A, a, b, c, x0, y0, z0 in PHANTOM:
+        a, b, c = a * half, b * half, c * half
+        q = 2 * np.pi * np.sqrt((a * k[:, 0]) ** 2 + (b * k[:, 1]) ** 2 + (c * k[:, 2]) ** 2)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            f = np.where(q > 1e-6, 4 * np.pi * (np.sin(q) - q * np.cos(q)) / np.maximum(q, 1e-30) ** 3,
+                         4 * np.pi / 3)
+        ph = np.exp(-2j * np.pi * (k[:, 0] * x0 * half + k[:, 1] * y0 * half + k[:, 2] * z0 * half))
+        out += A * a * b * c * f * ph
+    return out.astype(np.complex64)
+
+
+def evaluate_phantom(ctx, mlp, traj, N, rc, image, ramp=None, iters=10):
+    """Single-coil phantom reconstructions with the iterative, learned and
+    ramp DCFs: NRMSE of learned and ramp against iterative."""
+    tn, _ = hip.normalize_traj(traj, N)
+    s, p = plan_for(ctx, tn, N, 1)
+    s.set_coils(phantom_kspace(traj, image)[None, :])
+    f = hip.FFT(ctx, 3, N, 1, +1)
+    g = hip.Grids(ctx, 3, N, 1)
+    hip.pipe_menon(p, s, iters, True)
+    im_it = recon_image(ctx, p, s, N, rc, f, g)
+    learned_dcf(ctx, mlp, s, N)
+    im_ml = recon_image(ctx, p, s, N, rc, f, g)
+    out = dict(nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it))
+    if ramp is not None:
+        s.set_dcf(ramp)
+        im_rp = recon_image(ctx, p, s, N, rc, f, g)
+        out.update(nrmse_ramp=nrmse(im_rp, im_it), nrmse_ramp_scaled=nrmse_scaled(im_rp, im_it))
+    for h in (g, f, p, s):
+        h.close()
     return out

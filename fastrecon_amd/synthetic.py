@@ -134,9 +134,11 @@ def ldcf_training_trajectories(count=200, seed=7, scale=1.0):
     for n in range(count):
         r = uniform(np.arange(4, dtype=np.uint64) + 4 * n, seed)
         if n % 2 == 0:
+            # half of the radial sets are uniform-density (alpha 1), the
+            # common clinical kooshball; most read 256 samples per spoke
             spokes = max(16, int(scale * (4000 + r[0] * 56000)))
-            readout = int(128 + 128 * (r[1] > 0.5))
-            alpha = 1.0 + 1.0 * r[2]
+            readout = int(128 + 128 * (r[1] > 0.25))
+            alpha = 1.0 if r[2] < 0.5 else 1.0 + 2.0 * (r[2] - 0.5)
             yield f"radial{n}", radial3d_vd(spokes, readout, alpha, seed=seed * 1000 + n)
         else:
             cones = max(4, int(scale * (16 + r[0] * 48)))

@@ -242,7 +242,7 @@ fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth
 
 /* --------------------------------------------- learned density compensation
  * Net-new (the reference has no DCF estimation).  Features per sample, 8
- * floats per row (6 used): |u|, azimuth/pi, polar/pi, log1p(n_1)/8,
+ * floats per row (6 used): log1p(|u| N/2)/5, azimuth/pi, polar/pi, log1p(n_1)/8,
  * log1p(n_2)/8, log1p(n_4)/8, with u = (t - (N-1)/2)/(N/2) for the samples'
  * normalised trajectory t and n_r the number of other samples whose nearest
  * cell lies within Chebyshev distance r cells.  feat_dev: M*8 floats. */

@@ -7,7 +7,7 @@ Definitions restated here, independently of the HIP kernels:
 features(t, N) for normalised trajectory points t (grid units, as after
 ReconData::normalizeTraj, ReconData.cpp:211-225), with cf = (N-1)/2, h = N/2:
   u_d   = (t_d - cf) / h                      (FP32)
-  f0    = |u|                                 radius
+  f0    = log1p(|u| N/2) / 5                  radius in cells, log-compressed
   f1    = atan2(u_y, u_x) / pi                azimuth
   f2    = acos(clamp(u_z/|u|)) / pi           polar (0.5 in 2D or at u = 0)
   cell  = clamp(floor(t_d + 0.5), 0, N-1)
@@ -47,7 +47,7 @@ def features(tn, N):
     else:
         r = np.sqrt(u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1])
     f = np.zeros((M, NF), np.float64)
-    f[:, 0] = r
+    f[:, 0] = np.log1p(r.astype(np.float64) * float(h)) / 5.0
     f[:, 1] = np.arctan2(u[:, 1].astype(np.float64), u[:, 0].astype(np.float64)) / np.pi
     if d == 3:
         with np.errstate(invalid="ignore", divide="ignore"):

@@ -44,7 +44,7 @@ def test_features_match_oracle(ctx, dim, N, M):
     for q, r in enumerate((1, 2, 4)):
         n_gpu = np.rint(np.expm1(f[:, 3 + q].astype(np.float64) * 8)).astype(np.int64)
         assert np.array_equal(n_gpu, lo.neighbour_counts(t, N, r))
-    np.testing.assert_allclose(f[:, 0], ref[:, 0], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(f[:, 0], ref[:, 0], rtol=2e-6, atol=1e-6)
     np.testing.assert_allclose(f[:, 1:3], ref[:, 1:3], atol=1e-5)
     assert np.all(f[:, 6:] == 0)
 

@@ -52,7 +52,8 @@ def test_angular_features_closed_forms():
     c = 16.0
     pts = np.array([[c + 8, c, c], [c, c + 8, c], [c, c, c + 8], [c, c - 8, c], [c - 8, c, c - 8]], np.float32)
     f = lo.features(pts, N)
-    np.testing.assert_allclose(f[:, 0], [8 / 16.5, 8 / 16.5, 8 / 16.5, 8 / 16.5, np.sqrt(2) * 8 / 16.5], rtol=1e-6)
+    # radius in cells (N/2 = 16.5 per unit of u), log-compressed
+    np.testing.assert_allclose(f[:, 0], np.log1p([8, 8, 8, 8, np.sqrt(2) * 8]) / 5, rtol=1e-6)
     np.testing.assert_allclose(f[:, 1], [0.0, 0.5, 0.0, -0.5, 1.0], atol=1e-7)
     np.testing.assert_allclose(f[:, 2], [0.5, 0.5, 0.0, 0.5, 0.75], atol=1e-7)
     assert np.all(f[:, 3:] == 0)           # isolated points: no neighbours
