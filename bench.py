@@ -298,8 +298,8 @@ def main():
                    "samples_per_coil": M, "coils": C, "grid": N,
                    "tile": [info["tile_x"], info["tile_y"], info["tile_z"]], "slabs": info["slabs"],
                    "accumulate": {0: "automatic (lds read-add-write)", 1: "lds float atomic add",
-                                  2: "lds read-add-write", 4: "gather (cell-sorted batches, vgpr cells)"}[
-                                      a.accumulate],
+                                  2: "lds read-add-write", 4: "gather (cell-sorted batches, vgpr cells)",
+                                  5: "matrix cores (16-cell blocks, weights on the fly)"}[a.accumulate],
                    "order": "exact (CPU summation order)" if a.exact else "sliced (fixed-order slice combine)",
                    "items": info["items"], "heavy_units": info["heavy_units"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

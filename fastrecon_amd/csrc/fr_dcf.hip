@@ -33,6 +33,55 @@ __global__ void k_degrid(const float *__restrict__ traj, int64_t M, int dim, int
     out[s * C + c] = make_float2(re, im);
 }
 
+// Same sums as k_degrid, with the LUT index taken from a fast square root
+// corrected by the plan's exact thresholds (B[m] <= d2 < B[m+1]; the tap test
+// d2 < dacc is sqrtf(d2) < hw): identical taps, indices and summation order,
+// so the result is bit-identical to k_degrid, at a fraction of its
+// arithmetic.  The thresholds and the LUT are staged in LDS.
+__global__ void __launch_bounds__(256) k_degrid_thr(const float *__restrict__ traj, int64_t M, int dim, int N,
+                                                    float hw, int L, const float *__restrict__ lut,
+                                                    const float2 *__restrict__ gpairs, float dacc, float cidx,
+                                                    const float2 *__restrict__ grid, size_t cells, int C,
+                                                    float2 *__restrict__ out, int frames)
+{
+    extern __shared__ float2 sm[];
+    float2 *pairs = sm;
+    float *slut = reinterpret_cast<float *>(sm + L + 1);
+    for (int q = threadIdx.x; q <= L; q += blockDim.x) pairs[q] = gpairs[q];
+    for (int q = threadIdx.x; q < L; q += blockDim.x) slut[q] = lut[q];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * C) return;
+    const int64_t s = i / C;
+    const int c = (int)(i - s * C);
+    fr_box b;
+    fr_sample_box(dim, N, traj, M, s, hw, b, frames);
+    const float2 *g = grid + (size_t)c * cells;
+    float re = 0.f, im = 0.f;
+    for (int z = b.s[2]; z <= b.e[2]; ++z) {
+        const float dz = __fsub_rn((float)z, b.c[2]);
+        const float dz2 = __fmul_rn(dz, dz);
+        for (int y = b.s[1]; y <= b.e[1]; ++y) {
+            const float dy = __fsub_rn((float)y, b.c[1]);
+            const float dy2 = __fmul_rn(dy, dy);
+            for (int x = b.s[0]; x <= b.e[0]; ++x) {
+                const float dx = __fsub_rn((float)x, b.c[0]);
+                const float d2 = fr_tap_dk2(dz2, dy2, dx);
+                if (d2 < dacc) {
+                    const int ka = min((int)__builtin_fmaf(__builtin_amdgcn_sqrtf(d2), cidx, 0.5f), L);
+                    const float2 bb = pairs[ka];
+                    const int ki = min(max(ka - (d2 < bb.x ? 1 : 0) + (d2 >= bb.y ? 1 : 0), 0), L - 1);
+                    const float w = slut[ki];
+                    const float2 v = g[((size_t)z * N + y) * N + x];
+                    re = __fadd_rn(re, __fmul_rn(w, v.x));
+                    im = __fadd_rn(im, __fmul_rn(w, v.y));
+                }
+            }
+        }
+    }
+    out[s * C + c] = make_float2(re, im);
+}
+
 __global__ void k_fill_unit(float2 *__restrict__ d, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -61,6 +110,14 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 fr_status fr_degrid_raw(fr_plan *p, const fr_samples *s, const float2 *grid, int C, float2 *out)
 {
     fr_ctx *ctx = p->ctx;
+    if (p->mfma_pairs) {
+        const size_t lds = sizeof(float2) * (p->L + 1) + sizeof(float) * p->L;
+        hipLaunchKernelGGL(k_degrid_thr, dim3(nblk(p->M * C, 256)), dim3(256), lds, ctx->stream, s->traj, p->M,
+                           p->dim, (int)p->N, p->kHW, p->L, p->lut, p->mfma_pairs, p->mfma_dacc, p->mfma_cidx, grid,
+                           fr_plan_cells(p->dim, p->N, p->frames), C, out, p->frames);
+        FR_HIP(ctx, hipGetLastError());
+        return FR_OK;
+    }
     hipLaunchKernelGGL(k_degrid, dim3(nblk(p->M * C, 256)), dim3(256), 0, ctx->stream, s->traj, p->M, p->dim,
                        (int)p->N, p->kHW, p->L, p->lut, grid, fr_plan_cells(p->dim, p->N, p->frames), C, out,
                        p->frames);
@@ -112,7 +169,11 @@ extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int
             hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
         u->has_traj = u->has_dcf = true;
         for (int it = 0; it < iters && !st; ++it) {
-            st = fr_launch_grid(p, u, g->data, 0);
+            // single-channel density gridding: the matrix-core kernel (any
+            // plan's records serve it), several times faster here than the
+            // LDS kernels, whose lanes are the coil components; an
+            // exact-order plan keeps its sequential mode (CPU summation order)
+            st = fr_launch_grid(p, u, g->data, p->mfma_pairs && !p->exact_order ? FR_ACC_MFMA : 0);
             if (!st) st = fr_degrid_raw(p, u, g->data, 1, den);
             if (!st) hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, den, M);
         }

@@ -435,19 +435,6 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
     const int acc_floats = ((g.cells_max + 1) * stride + 3) & ~3;
     const size_t lds = (size_t)acc_floats * sizeof(float) + sizeof(uint2) * 64 * STAGE;
     const int chunks = (s->C + 31) / 32;
-    if (p->nslots > 0) {
-        const size_t need = (size_t)chunks * p->nslots * 32 * g.cells_max * sizeof(float2);
-        if (need > p->partial_bytes) {
-            FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            hipFree(p->partial);
-            p->partial = nullptr;
-            p->partial_bytes = 0;
-            fr_status e = fr_alloc(ctx, (void **)&p->partial, need);
-            if (e) return e;
-            p->partial_bytes = need;
-        }
-    }
-    if (p->nitems == 0) return FR_OK;
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int64_t light = p->nitems - p->nitems_front;
     const int64_t span = 8 * XCD_RUN;
@@ -465,27 +452,54 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
                            (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
     }
     FR_HIP(ctx, hipGetLastError());
-    if (p->nheavy > 0) {
-        hipLaunchKernelGGL(k_combine, dim3((unsigned)p->nheavy, chunks), dim3(256), 0, ctx->stream, p->heavy,
-                           p->units, (const float2 *)p->partial, p->nslots, g.cells_max, s->C, (int)p->N, cells, out);
-        FR_HIP(ctx, hipGetLastError());
-    }
     return FR_OK;
 }
 
 }  // namespace
 
-fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int)
+fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode)
 {
-    const bool c64 = s->type == FR_SAMPLES_C64;
-    switch (p->accumulate) {
-    case FR_ACC_GATHER:
-        return c64 ? launch<FR_ACC_GATHER, float2>(p, s, out) : launch<FR_ACC_GATHER, __half2>(p, s, out);
-    case FR_ACC_LDS_ATOMIC:
-        return c64 ? launch<FR_ACC_LDS_ATOMIC, float2>(p, s, out) : launch<FR_ACC_LDS_ATOMIC, __half2>(p, s, out);
-    default:
-        return c64 ? launch<FR_ACC_LDS_RMW, float2>(p, s, out) : launch<FR_ACC_LDS_RMW, __half2>(p, s, out);
+    fr_ctx *ctx = p->ctx;
+    const fr_geom &g = p->geom;
+    // heavy-unit slices: partial boxes [32-coil chunk][slot][32 coils][cells_max]
+    const int chunks = (s->C + 31) / 32;
+    if (p->nslots > 0) {
+        const size_t need = (size_t)chunks * p->nslots * 32 * g.cells_max * sizeof(float2);
+        if (need > p->partial_bytes) {
+            FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            hipFree(p->partial);
+            p->partial = nullptr;
+            p->partial_bytes = 0;
+            fr_status e = fr_alloc(ctx, (void **)&p->partial, need);
+            if (e) return e;
+            p->partial_bytes = need;
+        }
     }
+    if (p->nitems == 0) return FR_OK;
+    const bool c64 = s->type == FR_SAMPLES_C64;
+    fr_status e;
+    switch (force_mode ? force_mode : p->accumulate) {
+    case FR_ACC_MFMA:
+        e = fr_launch_grid_mfma(p, s, out);
+        break;
+    case FR_ACC_GATHER:
+        e = c64 ? launch<FR_ACC_GATHER, float2>(p, s, out) : launch<FR_ACC_GATHER, __half2>(p, s, out);
+        break;
+    case FR_ACC_LDS_ATOMIC:
+        e = c64 ? launch<FR_ACC_LDS_ATOMIC, float2>(p, s, out) : launch<FR_ACC_LDS_ATOMIC, __half2>(p, s, out);
+        break;
+    default:
+        e = c64 ? launch<FR_ACC_LDS_RMW, float2>(p, s, out) : launch<FR_ACC_LDS_RMW, __half2>(p, s, out);
+        break;
+    }
+    if (e) return e;
+    if (p->nheavy > 0) {
+        const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
+        hipLaunchKernelGGL(k_combine, dim3((unsigned)p->nheavy, chunks), dim3(256), 0, ctx->stream, p->heavy,
+                           p->units, (const float2 *)p->partial, p->nslots, g.cells_max, s->C, (int)p->N, cells, out);
+        FR_HIP(ctx, hipGetLastError());
+    }
+    return FR_OK;
 }
 
 extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *out)

@@ -61,6 +61,9 @@ struct fr_grids {
 #define FR_ACC_GATHER 4     // cell-sorted batches, sample rows in LDS, VGPR accumulators
 #define FR_GATHER_BATCH 32  // records per gather batch
 #define FR_GATHER_CELLS 32  // unit cells held in VGPRs by the gather kernel
+#define FR_ACC_MFMA 5       // matrix-core products of 16-cell blocks, weights computed on the fly
+#define FR_MFMA_MAX_L 4096  // LUT entries the matrix-core kernel stages in LDS
+#define FR_MFMA_WAVES_PER_CU 8    // persistent waves of the matrix-core kernel per CU (measured best of 4-32)
 
 // One batch of a gather-mode item: records [r0, r0 + nrec) and their taps
 // T[toff ..) sorted by cell; cnt byte k = tap count of cell k (a multiple
@@ -138,6 +141,12 @@ struct fr_plan {
     float *partial = nullptr;          // scratch for heavy-unit slices
     size_t partial_bytes = 0;
     fr_batch *batches = nullptr;       // gather mode: item i owns batches [items[i].t0, ..)
+    // matrix-core mode: exact LUT-index thresholds on the squared distance,
+    // pairs (B[m], B[m+1]) for m = 0 .. L, B[m] = least d2 whose reference
+    // LUT index is >= m (fr_plan.hip, mfma_thresholds)
+    float2 *mfma_pairs = nullptr;
+    float mfma_dacc = 0.f;             // least d2 with sqrtf(d2) >= hw (rejected from here on)
+    float mfma_cidx = 0.f;             // approximately (L - 1) / hw
     int64_t nbatches = 0;
     double build_ms = 0.0;
 };
@@ -184,5 +193,8 @@ static inline size_t fr_plan_cells(int dim, unsigned N, int frames)
 }
 
 // Launch helpers implemented in the kernel translation units.
-fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int C_out_offset);
+// force_mode: 0 = the plan's accumulation mode, else that mode (any plan's
+// records serve FR_ACC_MFMA when its LUT has at most FR_MFMA_MAX_L entries)
+fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode);
+fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out);
 fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune);

@@ -26,6 +26,8 @@
 #include <algorithm>
 #include <chrono>
 #include <numeric>
+#include <cmath>
+#include <cstring>
 
 namespace {
 
@@ -172,6 +174,7 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
     const uint32_t n = (uint32_t)(vals[r] & 0xffffffffu);
     const uint32_t np = padded_taps(n);
     H[r] = make_uint2((uint32_t)s, np);
+    if (!T) return;                             // records only
     fr_box b;
     fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b, g.frames);
     uint2 *w = T + toff[r];
@@ -343,6 +346,33 @@ struct dbuf {
     ~dbuf() { if (p) (void)hipFree(p); }
 };
 
+// The reference's LUT index of a squared distance (GridLut.cpp:112-115):
+// dk = sqrtf(d2), roundf(dk / hw * (L - 1)), in IEEE FP32 on the host.
+int ref_lut_index(float d2, float hw, int L)
+{
+    const float dk = sqrtf(d2);
+    const float q = dk / hw;
+    const float r = q * (float)(L - 1);
+    return (int)roundf(r);
+}
+
+// Least non-negative float x with pred(x), by bisection on the bit pattern
+// (non-negative floats order like their bits); pred is monotone.
+template <typename P>
+float least_float(uint32_t hi_bits, P &&pred)
+{
+    uint32_t lo = 0, hi = hi_bits;               // pred(hi) holds
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        float x;
+        std::memcpy(&x, &mid, 4);
+        if (pred(x)) hi = mid; else lo = mid + 1;
+    }
+    float x;
+    std::memcpy(&x, &lo, 4);
+    return x;
+}
+
 int floor_pow2(int v)
 {
     int p = 1;
@@ -357,15 +387,24 @@ int floor_pow2(int v)
 // carries more than 64 taps.  Smaller units cost more records but use less
 // LDS per wave, so more waves per CU hide the LDS and memory latency; on
 // MI355X 16-32 cells per unit measured best for the 32-coil kooshball.
-static fr_geom choose_geom(int dim, unsigned N, int coils, const fr_tune *tune)
+//
+// The matrix-core mode (FR_ACC_MFMA) pays per (record, 16-cell block), so its
+// base unit is one block: 4 x 2 x 2 in 3D, 4 x 4 in 2D.
+static fr_geom choose_geom(int dim, unsigned N, int coils, int accumulate, const fr_tune *tune)
 {
     (void)coils;
     fr_geom g{};
     g.dim = dim;
     g.N = (int)N;
-    g.bx = dim == 3 ? 4 : 8;
-    g.by = 4;
-    g.bz = dim == 3 ? 2 : 1;
+    if (accumulate == FR_ACC_MFMA) {
+        g.bx = 4;
+        g.by = dim == 3 ? 2 : 4;
+        g.bz = dim == 3 ? 2 : 1;
+    } else {
+        g.bx = dim == 3 ? 4 : 8;
+        g.by = 4;
+        g.bz = dim == 3 ? 2 : 1;
+    }
     g.waves = 1;
     if (tune) {
         if (tune->tile_x > 0) g.bx = tune->tile_x;
@@ -431,7 +470,8 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     fr_ctx *ctx = p->ctx;
     hipStream_t st = ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
-    fr_geom geom = choose_geom(p->dim, p->N, s->C, tune);
+    int acc_mode = tune ? tune->accumulate : 0;
+    fr_geom geom = choose_geom(p->dim, p->N, s->C, acc_mode, tune);
     const int stride = 2 * std::min(s->C, 32) + 1;
     FR_CHECK(ctx, geom.bx >= 1 && geom.by >= 1 && geom.bz >= 1 && geom.bx <= 255 && geom.by <= 255 && geom.bz <= 255,
              FR_ERR_INVALID, "unit box %dx%dx%d out of range", geom.bx, geom.by, geom.bz);
@@ -439,9 +479,14 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
              geom.bz);
     (void)stride;
     p->geom = geom;
-    p->accumulate = tune ? tune->accumulate : 0;
-    FR_CHECK(ctx, p->accumulate >= 0 && p->accumulate <= FR_ACC_GATHER, FR_ERR_INVALID, "unknown accumulate mode %d",
+    p->accumulate = acc_mode;
+    FR_CHECK(ctx, p->accumulate >= 0 && p->accumulate <= FR_ACC_MFMA, FR_ERR_INVALID, "unknown accumulate mode %d",
              p->accumulate);
+    FR_CHECK(ctx, p->accumulate != FR_ACC_MFMA || !(tune && tune->exact_order), FR_ERR_INVALID,
+             "exact_order needs a sequential accumulation mode (1, 2 or 4); the matrix cores sum four products at "
+             "a time");
+    FR_CHECK(ctx, p->accumulate != FR_ACC_MFMA || p->L <= FR_MFMA_MAX_L, FR_ERR_UNSUPPORTED,
+             "matrix-core accumulation stages the LUT in LDS: L %d exceeds %d", p->L, FR_MFMA_MAX_L);
     FR_CHECK(ctx, p->accumulate != FR_ACC_GATHER || geom.cells_max <= FR_GATHER_CELLS, FR_ERR_INVALID,
              "gather accumulation needs a unit box of at most %d cells (have %dx%dx%d)", FR_GATHER_CELLS, geom.bx,
              geom.by, geom.bz);
@@ -457,6 +502,32 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     const int64_t M = s->M;
     gctx g{p->kHW, p->L, p->dim, N, p->frames};
     fr_status e;
+
+    if (p->L <= FR_MFMA_MAX_L) {
+        // (every plan: the matrix-core kernel also serves single-channel
+        // density gridding and degridding uses the same thresholds)
+        // The matrix-core kernel takes a fast (1 ulp) square root, whose
+        // index estimate is within one of the reference index, and corrects
+        // it exactly with these thresholds: index = m where B[m] <= d2 <
+        // B[m+1].  The index is monotone in d2, so the thresholds exist.
+        const float hw = p->kHW;
+        const int L = p->L;
+        const float top = 64.f * hw * hw + 64.f;        // far beyond every accepted tap
+        uint32_t top_bits;
+        std::memcpy(&top_bits, &top, 4);
+        std::vector<float> B(L + 2, INFINITY);
+        for (int m = 0; m < L; ++m) {
+            if (ref_lut_index(top, hw, L) < m) break;           // not reached below the search bound
+            B[m] = least_float(top_bits, [&](float d2) { return ref_lut_index(d2, hw, L) >= m; });
+        }
+        p->mfma_dacc = least_float(top_bits, [&](float d2) { return sqrtf(d2) >= hw; });
+        p->mfma_cidx = (float)((double)(L - 1) / (double)hw);
+        std::vector<float2> pairs(L + 1);
+        for (int m = 0; m <= L; ++m) pairs[m] = make_float2(B[m], B[m + 1]);
+        if ((e = fr_alloc(ctx, (void **)&p->mfma_pairs, sizeof(float2) * (L + 1)))) return e;
+        FR_HIP(ctx, hipMemcpyAsync(p->mfma_pairs, pairs.data(), sizeof(float2) * (L + 1), hipMemcpyHostToDevice, st));
+        FR_HIP(ctx, hipStreamSynchronize(st));
+    }
 
     // 1. per-cell tap histogram
     dbuf<uint32_t> cnt_d;
@@ -589,11 +660,14 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     FR_CHECK(ctx, TT + 1024 < (uint64_t(1) << 32), FR_ERR_UNSUPPORTED, "plan has %llu taps (max 2^32 - 1024)",
              (unsigned long long)TT);
 
-    // 5. unit record/tap ranges, then the records themselves
+    // 5. unit record/tap ranges, then the records themselves (the
+    // matrix-core kernel computes its weights and reads no tap stream)
+    const bool write_taps = p->accumulate != FR_ACC_MFMA;
+    const uint64_t tap_alloc = write_taps ? TT + 576 : 64;
     if ((e = fr_alloc(ctx, (void **)&p->hdrs, sizeof(uint2) * (R + 64)))) return e;
     // padded so a wave may stage 512 taps from any 64-aligned chunk base
-    if ((e = fr_alloc(ctx, (void **)&p->taps, sizeof(uint2) * (TT + 576)))) return e;
-    FR_HIP(ctx, hipMemsetAsync(p->taps, 0, sizeof(uint2) * (TT + 576), st));
+    if ((e = fr_alloc(ctx, (void **)&p->taps, sizeof(uint2) * tap_alloc))) return e;
+    FR_HIP(ctx, hipMemsetAsync(p->taps, 0, sizeof(uint2) * tap_alloc, st));
     FR_HIP(ctx, hipMemsetAsync(p->hdrs, 0, sizeof(uint2) * (R + 64), st));
     // tap cell field: the byte offset of the cell's accumulator row; padding
     // taps go to the spare row cells_max with weight 0
@@ -607,7 +681,8 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
                            p->units);
         hipLaunchKernelGGL(k_unit_counts, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, toff.p);
         hipLaunchKernelGGL(k_fill_records, dim3(nblk(R, 128)), dim3(128), 0, st, s->traj, M, g, cu_d.p, p->units,
-                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, tap_scale, pad_x, p->hdrs, p->taps);
+                           keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, tap_scale, pad_x, p->hdrs,
+                           write_taps ? p->taps : nullptr);
     }
     FR_HIP(ctx, hipGetLastError());
 
@@ -763,7 +838,7 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
     if (!p) return FR_OK;
     hipStreamSynchronize(p->ctx->stream);
     hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps);
-    hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches);
+    hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches); hipFree(p->mfma_pairs);
     delete p;
     return FR_OK;
 }
@@ -779,7 +854,8 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->slabs = p->geom.waves;
     info->tiles = (int)std::min<int64_t>(p->nunits, INT32_MAX);
     info->lds_bytes = 0;
-    info->plan_bytes = (int64_t)sizeof(uint2) * (p->nrecords + p->ntaps) + (int64_t)sizeof(fr_unit) * p->nunits;
+    info->plan_bytes = (int64_t)sizeof(uint2) * (p->nrecords + (p->accumulate == FR_ACC_MFMA ? 0 : p->ntaps)) +
+                       (int64_t)sizeof(fr_unit) * p->nunits;
     info->build_ms = p->build_ms;
     info->items = p->nitems;
     info->heavy_units = p->nheavy;

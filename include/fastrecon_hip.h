@@ -62,7 +62,14 @@ typedef struct {
                                     3: VGPRs -- refused (FR_ERR_UNSUPPORTED),
                                     4: gather -- per-batch cell-sorted taps, the
                                        batch's sample rows in LDS, one VGPR per
-                                       cell (unit box of at most 32 cells) */
+                                       cell (unit box of at most 32 cells),
+                                    5: matrix cores -- 16-cell blocks of a unit
+                                       accumulate four records per
+                                       v_mfma_f32_16x16x4_f32 with tap weights
+                                       computed on the fly (reference tap set and
+                                       LUT index, exact); sums group four
+                                       products, so exact_order is refused;
+                                       L <= 4096; no tap stream is stored */
     int split_taps;              /* heavier units are split (0: automatic)      */
     int exact_order;             /* 1: split units by space only, so every cell
                                     sums in ascending sample order (bit-identical

@@ -253,6 +253,47 @@ void or_degrid_f64(int dim, unsigned N, int64_t M, int C,
     }
 }
 
+/* The same degridding sum in FP32, accumulated in the tap order above
+ * (z, y, x), one rounding per multiply and per add: the arithmetic of the
+ * device degridding kernels (net-new: the reference has no degridding). */
+void or_degrid_f32(int dim, unsigned N, int64_t M, int C,
+                   const float *traj, int stride,
+                   const float *grid, float kWidth, const float *lut, int L,
+                   float *out)
+{
+    const float kHW = kWidth / 2.0f;
+    const size_t cells = (dim == 3) ? (size_t)N * N * N : (size_t)N * N;
+    for (int c = 0; c < C; ++c) {
+        const float *g = grid + 2 * cells * (size_t)c;
+        float *o = out + 2 * (size_t)M * (size_t)c;
+        for (int64_t s = 0; s < M; ++s) {
+            box_t b;
+            sample_box(dim, N, traj + s * stride, kHW, &b);
+            float are = 0.0f, aim = 0.0f;
+            for (int z = b.start[2]; z <= b.end[2]; ++z) {
+                float dz = (float)z - b.center[2];
+                float dz2 = dz * dz;
+                for (int y = b.start[1]; y <= b.end[1]; ++y) {
+                    float dy = (float)y - b.center[1];
+                    float dy2 = dy * dy;
+                    for (int x = b.start[0]; x <= b.end[0]; ++x) {
+                        float dx = (float)x - b.center[0];
+                        float dk = sqrtf((dz2 + dy2) + dx * dx);
+                        if (dk < kHW) {
+                            float w = lut[tap_index(dk, kHW, L)];
+                            size_t cell = (size_t)z * N * N + (size_t)y * N + (size_t)x;
+                            are = are + w * g[2 * cell];
+                            aim = aim + w * g[2 * cell + 1];
+                        }
+                    }
+                }
+            }
+            o[2 * s] = are;
+            o[2 * s + 1] = aim;
+        }
+    }
+}
+
 /* ---------------------------------------------------------------- FFT */
 /* Mixed-radix decimation-in-time FFT in double, unnormalised,
  * X[k] = sum_n x[n] exp(sign * 2 pi i n k / n_len). */

@@ -28,6 +28,7 @@
  *   or_sos_map       -- map == data gives SOS^2 (real), conj-linear
  *                       in the map                                 (pinned)
  *   or_degrid_f64    -- adjoint identity against or_grid_f64        (pinned)
+ *   or_degrid_f32    -- FP64 degrid to FP32 rounding, one-tap exact  (pinned)
  */
 #ifndef FR_ORACLE_H
 #define FR_ORACLE_H
@@ -68,6 +69,12 @@ void or_grid_f64(int dim, unsigned N, int64_t M, int C,
                  const float *traj, int stride, const float *dcf,
                  const float *data, float kWidth, const float *lut, int L,
                  double *out, int nthreads);
+
+/* FP32 degridding in tap order (the device kernels' arithmetic). */
+void or_degrid_f32(int dim, unsigned N, int64_t M, int C,
+                   const float *traj, int stride,
+                   const float *grid, float kWidth, const float *lut, int L,
+                   float *out);
 
 /* Adjoint of or_grid_f64 (degridding): out[c][s] = sum_taps w * grid[c][cell]
  * (no dcf).  Same taps as gridding. */

@@ -41,6 +41,8 @@ def lib():
                                   _f32p, c.c_float, _f32p, c.c_int, _f64p, c.c_int]
         L.or_degrid_f64.argtypes = [c.c_int, c.c_uint, c.c_int64, c.c_int, _f32p, c.c_int,
                                     _f64p, c.c_float, _f32p, c.c_int, _f64p]
+        L.or_degrid_f32.argtypes = [c.c_int, c.c_uint, c.c_int64, c.c_int, _f32p, c.c_int,
+                                    _f32p, c.c_float, _f32p, c.c_int, _f32p]
         L.or_fft_c2c.argtypes = [c.c_int, c.c_uint, c.c_int, _f64p, c.c_int]
         L.or_fftshift_f32.argtypes = [c.c_int, c.c_uint, c.c_int, _f32p]
         L.or_fftshift_f64.argtypes = [c.c_int, c.c_uint, c.c_int, _f64p]
@@ -122,6 +124,18 @@ def degrid_f64(traj, grid, N, kwidth, lut):
     lib().or_degrid_f64(dim, N, M, C, traj, dim, grid.view(np.float64).reshape(-1), kwidth,
                         np.ascontiguousarray(lut, np.float32), len(lut), out)
     return out.view(np.complex128).reshape(C, M)
+
+
+def degrid_f32(traj, grid, N, kwidth, lut):
+    """FP32 degridding in tap order (fr_oracle.c or_degrid_f32)."""
+    traj = np.ascontiguousarray(traj, np.float32)
+    M, dim = traj.shape
+    grid = np.ascontiguousarray(grid, np.complex64)
+    C = grid.shape[0]
+    out = np.zeros(C * M * 2, np.float32)
+    lib().or_degrid_f32(dim, N, M, C, traj, dim, grid.view(np.float32).reshape(-1), kwidth,
+                        np.ascontiguousarray(lut, np.float32), len(lut), out)
+    return out.view(np.complex64).reshape(C, M)
 
 
 def fft_c2c(dim, N, data, sign=+1):

@@ -128,6 +128,27 @@ def test_pipe_menon_on_batched_frames(ctx):
         o += tn.shape[0]
 
 
+def test_pipe_menon_on_batched_frames_sliced_plan(ctx):
+    """Default (sliced) plans run the density gridding on the matrix cores,
+    whose sums group records four at a time: batched and per-frame results
+    then agree to FP32 rounding rather than bitwise."""
+    N, lut, tns, dcfs, datas, image = frames_case(C=1, F=4)
+    s, sizes = batched_samples(ctx, tns, dcfs, datas, 1)
+    p = hip.Plan(ctx, s, N, 4.0, lut)
+    hip.pipe_menon(p, s, 3, True)
+    wb = s.get_dcf()
+    o = 0
+    for f, tn in enumerate(tns):
+        s1 = hip.Samples(ctx, 2, tn.shape[0], 1)
+        s1.set_traj(tn); s1.set_dcf(dcfs[f])
+        p1 = hip.Plan(ctx, s1, N, 4.0, lut)
+        hip.pipe_menon(p1, s1, 3, True)
+        w1 = s1.get_dcf()
+        e = np.linalg.norm(wb[o:o + tn.shape[0]] - w1) / np.linalg.norm(w1)
+        assert e < 1e-6, e
+        o += tn.shape[0]
+
+
 def test_autotune_picks_a_working_candidate(ctx, orc):
     C = 3
     N, lut, tns, dcfs, datas, image = frames_case(C=C, F=4)

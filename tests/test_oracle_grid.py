@@ -157,6 +157,33 @@ def test_degrid_is_adjoint(orc, lut):
         assert lhs == pytest.approx(rhs, rel=1e-12)
 
 
+def test_degrid_f32_pins(orc, lut):
+    """or_degrid_f32 equals the FP64 degridding to FP32 rounding, and on a
+    grid with one non-zero cell it returns exactly fl(LUT[idx] * v), with idx
+    from an FP64 evaluation of the tap distance at centres away from index-
+    rounding boundaries."""
+    rng = np.random.default_rng(12)
+    for dim, N, M in [(2, 24, 200), (3, 12, 80)]:
+        traj = rng.uniform(-0.5, N - 0.5, size=(M, dim)).astype(np.float32)
+        y = (rng.standard_normal((2, N ** dim)) + 1j * rng.standard_normal((2, N ** dim))).astype(np.complex64)
+        got = orc.degrid_f32(traj, y, N, 4.0, lut)
+        ref = orc.degrid_f64(traj, y.astype(np.complex128), N, 4.0, lut)
+        e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert e < 2e-7, e
+    for dim in (2, 3):
+        N = 16
+        c = _safe_center(dim, rng, N)
+        cell = np.floor(c).astype(int)                    # a cell at distance < sqrt(dim) < 2
+        v = np.complex64(0.7 - 1.3j)
+        y = np.zeros((1, N ** dim), np.complex64)
+        flat = int(np.ravel_multi_index(tuple(cell[::-1]), (N,) * dim))
+        y[0, flat] = v
+        out = orc.degrid_f32(c[None, :], y, N, 4.0, lut)[0, 0]
+        r = np.sqrt(np.sum((cell.astype(np.float64) - c.astype(np.float64)) ** 2))
+        w = np.float32(lut[_idx64(r)])
+        assert out.real == np.float32(w * np.float32(v.real)) and out.imag == np.float32(w * np.float32(v.imag))
+
+
 def test_normalize_traj_affine(orc):
     traj, _ = syn.radial2d(40, 64)
     N = 128
