@@ -80,16 +80,18 @@ extern "C" void fr_graph_destroy(fr_graph *gr)
     delete gr;
 }
 
-// Six default candidates: unit boxes around the
-// default (16-64 cells) with LDS read-add-write accumulation, and the 32-cell
-// boxes with the gather kernel (cell-sorted batches, VGPR cells).
+// Default candidates: unit boxes around the LDS default (16-64 cells) with
+// read-add-write accumulation, 32-cell boxes with the gather kernel, and
+// x-elongated 16/32-cell boxes on the matrix cores.
 static std::vector<fr_tune> default_candidates(int dim)
 {
     std::vector<fr_tune> c;
-    const int shapes3[][4] = {{4, 4, 2, 2}, {4, 2, 2, 2}, {4, 4, 4, 2}, {8, 4, 2, 2}, {4, 4, 2, 4}, {2, 4, 4, 4}};
-    const int shapes2[][4] = {{8, 4, 1, 2}, {4, 4, 1, 2}, {8, 8, 1, 2}, {16, 4, 1, 2}, {8, 4, 1, 4}, {4, 8, 1, 4}};
+    const int shapes3[][4] = {{4, 4, 2, 2}, {4, 2, 2, 2}, {8, 4, 2, 2}, {4, 4, 2, 4},
+                              {8, 2, 2, 5}, {8, 2, 1, 5}, {4, 2, 2, 5}, {8, 1, 4, 5}};
+    const int shapes2[][4] = {{8, 4, 1, 2}, {4, 4, 1, 2}, {16, 4, 1, 2}, {8, 4, 1, 4},
+                              {8, 4, 1, 5}, {16, 2, 1, 5}, {8, 2, 1, 5}, {4, 4, 1, 5}};
     const int(*shapes)[4] = dim == 3 ? shapes3 : shapes2;
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < 8; ++i) {
         fr_tune t{};
         t.tile_x = shapes[i][0];
         t.tile_y = shapes[i][1];

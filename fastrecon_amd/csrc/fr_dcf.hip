@@ -169,11 +169,12 @@ extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int
             hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
         u->has_traj = u->has_dcf = true;
         for (int it = 0; it < iters && !st; ++it) {
-            // single-channel density gridding: the matrix-core kernel (any
-            // plan's records serve it), several times faster here than the
-            // LDS kernels, whose lanes are the coil components; an
-            // exact-order plan keeps its sequential mode (CPU summation order)
-            st = fr_launch_grid(p, u, g->data, p->mfma_pairs && !p->exact_order ? FR_ACC_MFMA : 0);
+            // single-channel density gridding: the matrix-core kernel (the
+            // records and taps of plans in modes 1, 2 and 5 serve it), several
+            // times faster here than the LDS kernels, whose lanes are the coil
+            // components; an exact-order plan keeps its sequential mode (CPU
+            // summation order)
+            st = fr_launch_grid(p, u, g->data, p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
             if (!st) st = fr_degrid_raw(p, u, g->data, 1, den);
             if (!st) hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, den, M);
         }

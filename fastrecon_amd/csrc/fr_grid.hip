@@ -84,17 +84,22 @@ __device__ __forceinline__ void prep(batch_regs<TS> &B, uint2 &h, uint32_t &nb, 
                                      const TS *__restrict__ data, const float *__restrict__ dcf, int C, int lc,
                                      uint32_t vz, int lane)
 {
-    // h holds the headers of records nb .. nb + 63 (lane l: record nb + l)
+    // h holds the headers of records nb .. nb + 63 (lane l: record nb + l).
+    // The batch takes the longest run of records (at most BATCH) whose taps
+    // fit the stage: an inclusive prefix sum of the tap counts over lanes
+    // 0 .. BATCH-1 (within one 16-lane row), then a ballot.
     const uint32_t cb = ntb & ~63u;
-    int k = 0;
-    uint32_t te = ntb;
-    for (int j = 0; j < BATCH; ++j) {
-        if (nb + (uint32_t)j >= re) break;
-        const uint32_t n = rl(h.y, j);
-        if (te + n - cb > 64 * STAGE) break;   // records carry at most 64 taps, so k >= 1
-        te += n;
-        ++k;
+    const bool live = lane < BATCH && nb + (uint32_t)lane < re;
+    uint32_t pre = live ? h.y : 0u;
+#pragma unroll
+    for (int d = 1; d < BATCH; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)pre, d, BATCH);
+        if ((lane & (BATCH - 1)) >= d) pre += t;
     }
+    // records carry at most 64 taps and the stage holds 64 STAGE, so k >= 1
+    const bool fits = live && (ntb - cb) + pre <= 64u * STAGE;
+    const int k = __popcll(__ballot(fits));
+    const uint32_t te = ntb + (k > 0 ? rl(pre, k - 1) : 0u);
     B.k = k;
     B.cb = cb;
     B.tb = ntb;
@@ -129,6 +134,9 @@ __device__ __forceinline__ void compute(const batch_regs<TS> &B, uint2 *stage, c
         const float w = __uint_as_float(rl(__float_as_uint(B.w), j));
         const float d = __fmul_rn(w, B.d[j]);              // dcf * sample (GridLut.cpp:109)
         const uint2 *tp = stage + off;
+        // records average under two groups: a plain loop (no unrolled body
+        // plus remainder) keeps the per-record set-up short
+#pragma clang loop unroll(disable)
         for (uint32_t q = 0; q < n; q += FR_TAP_GROUP) {
             float *p[FR_TAP_GROUP];
             float v[FR_TAP_GROUP], x[FR_TAP_GROUP];

@@ -58,121 +58,131 @@ __device__ __forceinline__ float4 ld_pair(const __half2 *__restrict__ row, int c
     return make_float4(a.x, a.y, b.x, b.y);
 }
 
-// Weight A[cell][record] of one (cell, sample) pair: the reference tap test
-// and LUT entry.  The index comes from the fast square root (within one of
-// the reference index) corrected by the exact thresholds B[m] <= d2 < B[m+1].
-__device__ __forceinline__ float pair_weight(float fx, float fy, float fz, float px, float py, float pz, bool ok,
-                                             const float2 *pairs, const float *slut, float dacc, float cidx, int L)
-{
-    const float dz = __fsub_rn(fz, pz);
-    const float dy = __fsub_rn(fy, py);
-    const float dx = __fsub_rn(fx, px);
-    const float d2 = fr_tap_dk2(__fmul_rn(dz, dz), __fmul_rn(dy, dy), dx);   // (dz^2 + dy^2) + dx^2
-    const bool tap = ok && d2 < dacc;                                        // sqrtf(d2) < hw
-    const float t = __builtin_amdgcn_sqrtf(d2);
-    int ka = (int)__builtin_fmaf(t, cidx, 0.5f);
-    ka = min(ka, L);
-    const float2 bb = pairs[ka];
-    int ki = ka - (d2 < bb.x ? 1 : 0) + (d2 >= bb.y ? 1 : 0);
-    ki = min(max(ki, 0), L - 1);                // (never binds for a tap; keeps the read in bounds)
-    const float w = slut[ki];                   // computed for every pair: no branch
-    return tap ? w : 0.f;
-}
+constexpr int RB = 16;                       // records per batch
+constexpr int TAP_ROW = 4 * FR_ACC_STRIDE;   // plan taps: (cell * TAP_ROW, LUT weight bits)
+// the pipeline reads headers up to 2 RB + 15 records past a live record; the
+// plan pads H by 64 records
+static_assert(2 * RB + 15 < 64, "record header padding");
 
-// One group of four records as lane (j, k) sees it: record k's sample
-// coordinates and DCF, and the sample's component slice of lane j.
-template <int NCB>
-struct group_regs {
-    float px, py, pz, w;
-    float2 a, b;                                // coil slots (NCB = 4: two coils; else one)
+// One batch of RB records in flight.  Lane l < RB of h/n/excl/w describes
+// record l of the batch; tap pass p gives lane l tap u = l % TPR of record
+// p * RPP + l / TPR (a record carries at most TPR = 16 NBLK taps); rows:
+// group q gives lane (j, k) the component slice of record 4 q + k.
+template <int NBLK>
+struct batch_st {
+    uint32_t s;                 // lane l < RB: sample of record l
+    uint32_t n, excl;           // its (padded) tap count and first tap within the batch
+    float w;                    // its DCF (0 past the item's end)
+    uint2 tp[4 * NBLK];
+    float2 a[4], b[4];
 };
 
-template <int NCB, int ZM, bool PAIR, typename TS>
-__device__ __forceinline__ void load_group(group_regs<NCB> &G, uint32_t s, const float *__restrict__ tx,
-                                           const float *__restrict__ ty, const float *__restrict__ tz, int frames,
-                                           const TS *__restrict__ data, const float *__restrict__ dcf, uint32_t C,
-                                           int ca, int cbb)
+template <int NBLK, int NCB, bool PAIR, typename TS>
+__device__ __forceinline__ uint32_t issue_batch(batch_st<NBLK> &S, uint2 h, uint32_t rb, uint32_t r1, uint32_t tbase,
+                                                const uint2 *__restrict__ T, const TS *__restrict__ data,
+                                                const float *__restrict__ dcf, uint32_t C, int ca, int cbb, int lane)
 {
-    G.px = tx[s];
-    G.py = ty[s];
-    if constexpr (ZM == 1) G.pz = tz[s];
-    else if constexpr (ZM == 2) G.pz = (float)min(max((int)tz[s], 0), frames - 1);
-    else G.pz = 0.f;
-    G.w = dcf[s];
-    const TS *row = data + (uint64_t)s * C;
-    if constexpr (NCB == 4 && PAIR) {
-        // coils ca, ca + 1 in one 16-byte load (C even, ca even: aligned)
-        const float4 v = ld_pair(row, ca);
-        G.a = make_float2(v.x, v.y);
-        G.b = make_float2(v.z, v.w);
-    } else {
-        G.a = ld_coil(row, ca);
-        if constexpr (NCB == 4) G.b = ld_coil(row, cbb);
+    constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
+    const bool live = lane < RB && rb + (uint32_t)lane < r1;
+    const uint32_t n = live ? h.y : 0u;
+    uint32_t pre = n;                           // inclusive prefix over lanes 0 .. RB-1
+#pragma unroll
+    for (int d = 1; d < RB; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)pre, d, RB);
+        if ((lane & (RB - 1)) >= d) pre += t;
     }
+    S.s = h.x;
+    S.n = n;
+    S.excl = pre - n;
+    const float wd = dcf[h.x];
+    S.w = live ? wd : 0.f;
+#pragma unroll
+    for (int p = 0; p < 4 * NBLK; ++p) {
+        const int rho = p * RPP + lane / TPR;
+        const uint32_t u = (uint32_t)(lane % TPR);
+        const uint32_t nr = (uint32_t)__shfl((int)n, rho);
+        const uint32_t off = (uint32_t)__shfl((int)(pre - n), rho);
+        S.tp[p] = T[u < nr ? tbase + off + u : tbase];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t sq = (uint32_t)__shfl((int)h.x, 4 * q + (lane >> 4));
+        const TS *row = data + (uint64_t)sq * C;
+        if constexpr (NCB == 4 && PAIR) {
+            const float4 v = ld_pair(row, ca);
+            S.a[q] = make_float2(v.x, v.y);
+            S.b[q] = make_float2(v.z, v.w);
+        } else {
+            S.a[q] = ld_coil(row, ca);
+            if constexpr (NCB == 4) S.b[q] = ld_coil(row, cbb);
+        }
+    }
+    return (uint32_t)__builtin_amdgcn_readlane((int)pre, RB - 1);
 }
 
 template <int NBLK, int NCB>
-__device__ __forceinline__ void apply_group(f4 (&acc)[NBLK][NCB], const group_regs<NCB> &G, bool ok,
-                                            const float (&fx)[NBLK], const float (&fy)[NBLK],
-                                            const float (&fz)[NBLK], const bool (&cok)[NBLK], bool v0ok, bool v1ok,
-                                            int j, const float2 *pairs, const float *slut, float dacc, float cidx,
-                                            int L)
+__device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st<NBLK> &S, float *tile, bool v0ok,
+                                            bool v1ok, int lane)
 {
-    // B = dcf * sample (GridLut.cpp:109).  Padding columns (coils past the
-    // chunk) and records past the item's end get a zero DCF here (their
-    // loads read valid rows); the records also get zero weights in A.
-    const float w0 = ok && v0ok ? G.w : 0.f;
-    float bv[NCB];
-    if constexpr (NCB == 4) {
-        const float w1 = ok && v1ok ? G.w : 0.f;
-        bv[0] = __fmul_rn(w0, G.a.x); bv[1] = __fmul_rn(w0, G.a.y);
-        bv[2] = __fmul_rn(w1, G.b.x); bv[3] = __fmul_rn(w1, G.b.y);
-    } else if constexpr (NCB == 2) {
-        bv[0] = __fmul_rn(w0, G.a.x); bv[1] = __fmul_rn(w0, G.a.y);
-    } else {
-        bv[0] = __fmul_rn(w0, (j & 1) ? G.a.y : G.a.x);
+    constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
+    constexpr int ROW = 16 * NBLK + 1;          // cells (and the padding taps' row) per record column
+    const int j = lane & 15, k = lane >> 4;
+    // A tile [record][cell]: zero, then every tap writes its LUT weight
+    // (distinct cells within a record; padding taps write 0 to row cells_max)
+#pragma unroll
+    for (int z0 = 0; z0 < RB * ROW; z0 += 64)
+        if (z0 + lane < RB * ROW) tile[z0 + lane] = 0.f;
+#pragma unroll
+    for (int p = 0; p < 4 * NBLK; ++p) {
+        const int rho = p * RPP + lane / TPR;
+        const uint32_t nr = (uint32_t)__shfl((int)S.n, rho);
+        if ((uint32_t)(lane % TPR) < nr)
+            tile[rho * ROW + (int)(S.tp[p].x / TAP_ROW)] = __uint_as_float(S.tp[p].y);
     }
 #pragma unroll
-    for (int b = 0; b < NBLK; ++b) {
-        const float aw = pair_weight(fx[b], fy[b], fz[b], G.px, G.py, G.pz, ok && cok[b], pairs, slut, dacc, cidx, L);
+    for (int q = 0; q < 4; ++q) {
+        const int rec = 4 * q + k;
+        // B = dcf * sample (GridLut.cpp:109); padding coil columns get 0
+        const float w = __shfl(S.w, rec);
+        const float w0 = v0ok ? w : 0.f;
+        float bv[NCB];
+        if constexpr (NCB == 4) {
+            const float w1 = v1ok ? w : 0.f;
+            bv[0] = __fmul_rn(w0, S.a[q].x); bv[1] = __fmul_rn(w0, S.a[q].y);
+            bv[2] = __fmul_rn(w1, S.b[q].x); bv[3] = __fmul_rn(w1, S.b[q].y);
+        } else if constexpr (NCB == 2) {
+            bv[0] = __fmul_rn(w0, S.a[q].x); bv[1] = __fmul_rn(w0, S.a[q].y);
+        } else {
+            bv[0] = __fmul_rn(w0, (j & 1) ? S.a[q].y : S.a[q].x);
+        }
 #pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) acc[b][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aw, bv[cb], acc[b][cb], 0, 0, 0);
+        for (int b = 0; b < NBLK; ++b) {
+            const float aw = tile[rec * ROW + 16 * b + j];
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+                acc[b][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aw, bv[cb], acc[b][cb], 0, 0, 0);
+        }
     }
 }
 
-constexpr int BG = 4;          // groups of four records per prefetched batch
-// the pipeline reads headers up to 8 BG + 15 records past a live record; the
-// plan pads H by 64 records
-static_assert(8 * BG + 15 < 64, "record header padding");
-
-// ZM: 0 = 2D, 1 = 3D, 2 = batched 2D frames (third trajectory row = frame).
+// NBLK blocks of 16 cells; NCB component blocks (chunks of 8 NCB coils).
 // PAIR: C even, so lane j's two coils (2j, 2j+1 of a 32-coil chunk) are one
 // aligned 16-byte (8-byte for FP16) load.
-template <int NBLK, int NCB, int ZM, bool PAIR, typename TS>
+template <int NBLK, int NCB, bool PAIR, typename TS>
 __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ items, int64_t nitems,
                                                   const fr_unit *__restrict__ units, const uint2 *__restrict__ H,
-                                                  const float *__restrict__ traj, int64_t M, int frames,
-                                                  const TS *__restrict__ data, const float *__restrict__ dcf, int C,
-                                                  int N, size_t cells, float2 *__restrict__ out,
-                                                  float2 *__restrict__ partial, int64_t nslots, int cells_max,
-                                                  const float *__restrict__ lut, const float2 *__restrict__ gpairs,
-                                                  int L, float dacc, float cidx)
+                                                  const uint2 *__restrict__ T, const TS *__restrict__ data,
+                                                  const float *__restrict__ dcf, int C, int N, size_t cells,
+                                                  float2 *__restrict__ out, float2 *__restrict__ partial,
+                                                  int64_t nslots, int cells_max)
 {
-    // LDS: threshold pairs [L + 1] | LUT [L], staged once per wave
-    extern __shared__ float2 smem[];
-    float2 *pairs = smem;
-    float *slut = reinterpret_cast<float *>(smem + L + 1);
+    __shared__ float tile[RB * (16 * NBLK + 1)];
     const int lane = threadIdx.x;
-    for (int q = lane; q <= L; q += 64) pairs[q] = gpairs[q];
-    for (int q = lane; q < L; q += 64) slut[q] = lut[q];
-    __syncthreads();
-
     const int j = lane & 15;                    // A row (cell in block) / B column (component)
     const int k = lane >> 4;                    // A column / B row (record in group)
     const int chunk = 8 * NCB;                  // coils per chunk
     const int c0 = blockIdx.y * chunk;
     const int nc = min(chunk, C - c0);
-    const float *tx = traj, *ty = traj + M, *tz = traj + 2 * M;
     // lane j's coils (NCB = 4: 2j, 2j+1; 2: j; 1: j/2), clamped to a valid
     // coil for the load and masked through the DCF
     const int cl0 = NCB == 4 ? 2 * j : (NCB == 2 ? j : j >> 1);
@@ -193,18 +203,6 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
         const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
         const int ncell = ux * uy * uz;
 
-        float fx[NBLK], fy[NBLK], fz[NBLK];
-        bool cok[NBLK];
-#pragma unroll
-        for (int b = 0; b < NBLK; ++b) {
-            const int q = 16 * b + j;
-            cok[b] = q < ncell;
-            const int qq = cok[b] ? q : 0;
-            fx[b] = (float)(U.ox + qq % ux);
-            fy[b] = (float)(U.oy + (qq / ux) % uy);
-            fz[b] = (float)(U.oz + qq / (ux * uy));
-        }
-
         f4 acc[NBLK][NCB];
 #pragma unroll
         for (int b = 0; b < NBLK; ++b)
@@ -212,49 +210,38 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
             for (int cb = 0; cb < NCB; ++cb) acc[b][cb] = f4{0.f, 0.f, 0.f, 0.f};
 
         if (it.r0 < it.r1) {
-            // Software pipeline over batches of 16 records (4 groups): while
-            // batch b is applied, batch b+1's coordinates and sample rows
-            // (64 rows per wave) and batch b+2's sample indices are in
-            // flight.  Records past the item's end read valid (padded)
-            // headers and are masked; groups wholly past it are skipped.
+            // Software pipeline over batches of RB records: while batch b is
+            // applied, batch b+1's taps, DCFs and sample rows and batch b+2's
+            // headers are in flight.  Records past the item's end read valid
+            // (padded) headers and are masked (no taps, zero DCF).
             const uint32_t r1 = it.r1;
-            uint32_t r = it.r0;
-            group_regs<NCB> A[BG], B[BG];
-            uint32_t hn[BG], h2[BG];
-#pragma unroll
-            for (int q = 0; q < BG; ++q) hn[q] = H[r + 4 * q + k].x;
-#pragma unroll
-            for (int q = 0; q < BG; ++q)
-                load_group<NCB, ZM, PAIR>(A[q], hn[q], tx, ty, tz, frames, data, dcf, (uint32_t)C, ca, cbb);
-#pragma unroll
-            for (int q = 0; q < BG; ++q) hn[q] = H[r + 4 * BG + 4 * q + k].x;
+            uint32_t r = it.r0, tb = it.t0;
+            const uint32_t hl = (uint32_t)(lane & (RB - 1));
+            batch_st<NBLK> S0, S1;
+            uint32_t T0 = issue_batch<NBLK, NCB, PAIR>(S0, H[r + hl], r, r1, tb, T, data, dcf, (uint32_t)C, ca, cbb,
+                                                       lane);
+            uint2 h1 = H[r + RB + hl], h3;
+            uint32_t T1;
             while (true) {
-#pragma unroll
-                for (int q = 0; q < BG; ++q) h2[q] = H[r + 8 * BG + 4 * q + k].x;
-#pragma unroll
-                for (int q = 0; q < BG; ++q)
-                    load_group<NCB, ZM, PAIR>(B[q], hn[q], tx, ty, tz, frames, data, dcf, (uint32_t)C, ca, cbb);
+                const uint2 h2 = H[r + 2 * RB + hl];
+                T1 = issue_batch<NBLK, NCB, PAIR>(S1, h1, r + RB, r1, tb + T0, T, data, dcf, (uint32_t)C, ca, cbb,
+                                                  lane);
                 // keep the prefetch ahead of the compute (the scheduler would
                 // otherwise sink each load to its use and wait on it there)
                 __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int q = 0; q < BG; ++q)
-                    apply_group<NBLK, NCB>(acc, A[q], r + 4 * q + k < r1, fx, fy, fz, cok, v0ok, v1ok, j, pairs,
-                                           slut, dacc, cidx, L);
-                r += 4 * BG;
+                apply_batch<NBLK, NCB>(acc, S0, tile, v0ok, v1ok, lane);
+                r += RB;
+                tb += T0;
                 if (r >= r1) break;
-#pragma unroll
-                for (int q = 0; q < BG; ++q) hn[q] = H[r + 8 * BG + 4 * q + k].x;
-#pragma unroll
-                for (int q = 0; q < BG; ++q)
-                    load_group<NCB, ZM, PAIR>(A[q], h2[q], tx, ty, tz, frames, data, dcf, (uint32_t)C, ca, cbb);
+                h3 = H[r + 2 * RB + hl];
+                T0 = issue_batch<NBLK, NCB, PAIR>(S0, h2, r + RB, r1, tb + T1, T, data, dcf, (uint32_t)C, ca, cbb,
+                                                  lane);
                 __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int q = 0; q < BG; ++q)
-                    apply_group<NBLK, NCB>(acc, B[q], r + 4 * q + k < r1, fx, fy, fz, cok, v0ok, v1ok, j, pairs,
-                                           slut, dacc, cidx, L);
-                r += 4 * BG;
+                apply_batch<NBLK, NCB>(acc, S1, tile, v0ok, v1ok, lane);
+                r += RB;
+                tb += T1;
                 if (r >= r1) break;
+                h1 = h3;
             }
         }
 
@@ -291,7 +278,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     }
 }
 
-template <int NBLK, int NCB, int ZM, bool PAIR, typename TS>
+template <int NBLK, int NCB, bool PAIR, typename TS>
 fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out)
 {
     fr_ctx *ctx = p->ctx;
@@ -300,30 +287,20 @@ fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out)
     // persistent waves: a multiple of 8 (one share per XCD), at most one per item
     const int64_t want = (int64_t)std::max(1, ctx->cu_count) * FR_MFMA_WAVES_PER_CU;
     const int64_t W = std::max<int64_t>(8, std::min<int64_t>(want, (p->nitems + 7) / 8 * 8));
-    const size_t lds = sizeof(float2) * (p->L + 1) + sizeof(float) * p->L;
-    hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, ZM, PAIR, TS>), dim3((unsigned)W, chunks), dim3(64), lds, ctx->stream,
-                       p->items, p->nitems, p->units, p->hdrs, s->traj, s->M, p->frames, (const TS *)s->data, s->dcf,
-                       s->C, (int)p->N, cells, out, (float2 *)p->partial, p->nslots, p->geom.cells_max, p->lut,
-                       p->mfma_pairs, p->L, p->mfma_dacc, p->mfma_cidx);
+    hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
+                       p->items, p->nitems, p->units, p->hdrs, p->taps, (const TS *)s->data, s->dcf, s->C, (int)p->N,
+                       cells, out, (float2 *)p->partial, p->nslots, p->geom.cells_max);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
-}
-
-template <int NBLK, int NCB, bool PAIR, typename TS>
-fr_status launch_zm(fr_plan *p, const fr_samples *s, float2 *out)
-{
-    if (p->frames > 0) return launch_mfma<NBLK, NCB, 2, PAIR, TS>(p, s, out);
-    if (p->dim == 3) return launch_mfma<NBLK, NCB, 1, PAIR, TS>(p, s, out);
-    return launch_mfma<NBLK, NCB, 0, PAIR, TS>(p, s, out);
 }
 
 template <int NBLK, typename TS>
 fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out)
 {
-    if (s->C <= 8) return launch_zm<NBLK, 1, false, TS>(p, s, out);
-    if (s->C <= 16) return launch_zm<NBLK, 2, false, TS>(p, s, out);
-    if (s->C % 2 == 0) return launch_zm<NBLK, 4, true, TS>(p, s, out);
-    return launch_zm<NBLK, 4, false, TS>(p, s, out);
+    if (s->C <= 8) return launch_mfma<NBLK, 1, false, TS>(p, s, out);
+    if (s->C <= 16) return launch_mfma<NBLK, 2, false, TS>(p, s, out);
+    if (s->C % 2 == 0) return launch_mfma<NBLK, 4, true, TS>(p, s, out);
+    return launch_mfma<NBLK, 4, false, TS>(p, s, out);
 }
 
 template <typename TS>

@@ -388,8 +388,9 @@ int floor_pow2(int v)
 // LDS per wave, so more waves per CU hide the LDS and memory latency; on
 // MI355X 16-32 cells per unit measured best for the 32-coil kooshball.
 //
-// The matrix-core mode (FR_ACC_MFMA) pays per (record, 16-cell block), so its
-// base unit is one block: 4 x 2 x 2 in 3D, 4 x 4 in 2D.
+// The matrix-core mode (FR_ACC_MFMA) pays per (record, 16-cell block) and
+// writes rows of ux cells per coil: its base unit is 8 x 2 x 2 in 3D, 8 x 4 in
+// 2D (x-elongated shapes measured fastest on MI355X).
 static fr_geom choose_geom(int dim, unsigned N, int coils, int accumulate, const fr_tune *tune)
 {
     (void)coils;
@@ -397,7 +398,7 @@ static fr_geom choose_geom(int dim, unsigned N, int coils, int accumulate, const
     g.dim = dim;
     g.N = (int)N;
     if (accumulate == FR_ACC_MFMA) {
-        g.bx = 4;
+        g.bx = 8;
         g.by = dim == 3 ? 2 : 4;
         g.bz = dim == 3 ? 2 : 1;
     } else {
@@ -660,10 +661,9 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     FR_CHECK(ctx, TT + 1024 < (uint64_t(1) << 32), FR_ERR_UNSUPPORTED, "plan has %llu taps (max 2^32 - 1024)",
              (unsigned long long)TT);
 
-    // 5. unit record/tap ranges, then the records themselves (the
-    // matrix-core kernel computes its weights and reads no tap stream)
-    const bool write_taps = p->accumulate != FR_ACC_MFMA;
-    const uint64_t tap_alloc = write_taps ? TT + 576 : 64;
+    // 5. unit record/tap ranges, then the records themselves
+    const bool write_taps = true;
+    const uint64_t tap_alloc = TT + 576;
     if ((e = fr_alloc(ctx, (void **)&p->hdrs, sizeof(uint2) * (R + 64)))) return e;
     // padded so a wave may stage 512 taps from any 64-aligned chunk base
     if ((e = fr_alloc(ctx, (void **)&p->taps, sizeof(uint2) * tap_alloc))) return e;
@@ -854,8 +854,7 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->slabs = p->geom.waves;
     info->tiles = (int)std::min<int64_t>(p->nunits, INT32_MAX);
     info->lds_bytes = 0;
-    info->plan_bytes = (int64_t)sizeof(uint2) * (p->nrecords + (p->accumulate == FR_ACC_MFMA ? 0 : p->ntaps)) +
-                       (int64_t)sizeof(fr_unit) * p->nunits;
+    info->plan_bytes = (int64_t)sizeof(uint2) * (p->nrecords + p->ntaps) + (int64_t)sizeof(fr_unit) * p->nunits;
     info->build_ms = p->build_ms;
     info->items = p->nitems;
     info->heavy_units = p->nheavy;

@@ -580,6 +580,9 @@ class Graph:
             pass
 
 
+AUTOTUNE_DEFAULT_CANDIDATES = 8     # fr_autotune's built-in list (fr_batch.hip)
+
+
 def autotune(ctx, samples, N, kwidth, lut, candidates=None, reps=3):
     """Fastest fr_tune (as a dict) for these samples; also each candidate's ms."""
     lut = np.ascontiguousarray(lut, np.float32)
@@ -590,7 +593,7 @@ def autotune(ctx, samples, N, kwidth, lut, candidates=None, reps=3):
         cands = (fr_tune * n)(*[fr_tune(**{k: int(v) for k, v in c.items()}) for c in candidates])
     best = fr_tune()
     best_ms = C.c_float()
-    ms = np.zeros(max(n, 6), np.float32)
+    ms = np.zeros(max(n, AUTOTUNE_DEFAULT_CANDIDATES), np.float32)
     ctx.check(ctx.L.fr_autotune(ctx.h, samples.h, int(N), float(kwidth), _ptr(lut), len(lut), cands, n, int(reps),
                                 C.byref(best), C.byref(best_ms), _ptr(ms)), "fr_autotune")
     return {k: getattr(best, k) for k, _ in fr_tune._fields_}, best_ms.value, ms

@@ -247,6 +247,17 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     im_ml = img.to_host(np.float32)
     w_ml = s.get_dcf()
     t_rc = timed(recon)
+
+    def learned_refined():
+        # the learned DCF as the start of one Pipe-Menon iteration
+        feat = hip.ldcf_features(ctx, s, N)
+        mlp.dcf(feat, s)
+        hip.pipe_menon(p, s, 1, False)
+        recon()
+        feat.close()
+
+    t_mr = timed(learned_refined)
+    im_mr = img.to_host(np.float32)
     # one Pipe-Menon iteration (1 / kernel density) for reference
     hip.pipe_menon(p, s, 1, True)
     recon()
@@ -254,6 +265,8 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     out = dict(ms_iterative=t_it, ms_learned=t_ml, ms_recon_only=t_rc, speedup=t_it / t_ml,
                nrmse_pm1=nrmse(im_pm1, im_it), nrmse_pm1_scaled=nrmse_scaled(im_pm1, im_it),
                nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
+               ms_learned_pm1=t_mr, speedup_learned_pm1=t_it / t_mr,
+               nrmse_learned_pm1=nrmse(im_mr, im_it), nrmse_learned_pm1_scaled=nrmse_scaled(im_mr, im_it),
                dcf_rel_rms=float(np.sqrt(np.mean((w_ml / np.maximum(w_it, 1e-30) - 1.0) ** 2))))
     if ramp is not None:
         s.set_dcf(ramp)
@@ -266,6 +279,16 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     ones.set_coils(np.ones((1, M), np.complex64))
     g1 = hip.Grids(ctx, tn.shape[1], N, 1)
     f1 = hip.FFT(ctx, tn.shape[1], N, 1, +1)
+    # global scale from the Pipe-Menon fixed-point identity |G w|^2 = sum(w)
+    # (G^H G w = 1 at convergence): alpha = sum(w) / |G w|^2 for the learned w
+    ones.set_dcf(w_ml)
+    hip.grid_execute(p, ones, g1)
+    gw = g1.download(0)
+    alpha = float(np.sum(w_ml, dtype=np.float64) / np.sum(np.abs(gw.astype(np.complex128)) ** 2))
+    s.set_dcf((alpha * w_ml).astype(np.float32))
+    recon()
+    im_al = img.to_host(np.float32)
+    out.update(learned_identity_scale=alpha, nrmse_learned_identity_scaled=nrmse(im_al, im_it))
     for name, w in (("iterative", w_it), ("learned", w_ml)) + ((("ramp", ramp),) if ramp is not None else ()):
         ones.set_dcf(w)
         psf = recon_image(ctx, p, ones, N, rc, f1, g1)

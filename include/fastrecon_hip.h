@@ -236,10 +236,11 @@ fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *g, fr_fft *
                           unsigned rcz, int apply_shift, const float *deapod_dev, float *out_dev, fr_graph **out);
 fr_status fr_graph_launch(fr_graph *gr);
 void fr_graph_destroy(fr_graph *gr);
-/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: six
- * defaults -- unit shapes around the default with LDS read-add-write, and
- * 32-cell shapes with the gather kernel) on these samples and returns the
- * fastest in *best (and its time); cand_ms (optional, ncand or 6 entries)
+/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: eight
+ * defaults -- unit shapes around the default with LDS read-add-write, a
+ * 32-cell shape with the gather kernel, and x-elongated 16/32-cell shapes
+ * on the matrix cores) on these samples and returns the
+ * fastest in *best (and its time); cand_ms (optional, ncand or 8 entries)
  * gets each candidate's time, -1 for candidates the plan refuses.  The LUT length is
  * not tuned: tap weights are resolved when the plan is built, so L does not
  * change the pass time, and L = 512 keeps reference parity. */

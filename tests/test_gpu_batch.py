@@ -157,9 +157,16 @@ def test_autotune_picks_a_working_candidate(ctx, orc):
     print(f"[autotune] best {best} {ms:.3f} ms; candidates {cand.tolist()}", flush=True)
     assert ms > 0 and (cand > 0).sum() >= 4
     assert ms == pytest.approx(cand[cand > 0].min())
-    p = hip.Plan(ctx, s, N, 4.0, lut, dict(best, exact_order=1))
+    # the winner grids correctly; sequential modes also in exact CPU order
+    # (the matrix-core mode refuses exact_order: it sums four products at a time)
+    exact = best["accumulate"] != 5
+    p = hip.Plan(ctx, s, N, 4.0, lut, dict(best, exact_order=1) if exact else best)
     g = hip.Grids(ctx, 2, N, C, frames=len(tns))
     hip.grid_execute(p, s, g)
     got = g.download_all().reshape(C, len(tns), N * N)
     ref = orc.grid_f32(tns[0], dcfs[0], datas[0], N, 4.0, lut, nthreads=4)
-    assert np.array_equal(got[:, 0].view(np.int32), ref.view(np.int32))
+    if exact:
+        assert np.array_equal(got[:, 0].view(np.int32), ref.view(np.int32))
+    else:
+        e = np.linalg.norm(got[:, 0] - ref) / np.linalg.norm(ref)
+        assert e <= 1e-5, e
