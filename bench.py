@@ -70,9 +70,13 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_each_way": 4 * n, "launches": 5}), flush=True)
 
 
-def read_traffic(path, workload):
+GRID_KERNELS = {1: "k_grid_items (LDS atomic add)", 2: "k_grid_items (LDS read-add-write)",
+                4: "k_grid_gather (cell-sorted batches, VGPR cells)", 5: "k_grid_mfma (matrix cores)"}
+
+
+def read_traffic(path, workload, kernel):
     """HBM bytes per gridding pass from a committed counter summary, used
-    only when it was measured on this exact workload and kernel build."""
+    only when it was measured on this exact workload and kernel."""
     try:
         kv = {}
         with open(path) as f:
@@ -80,7 +84,7 @@ def read_traffic(path, workload):
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
                     kv[k.strip()] = v.strip()
-        if kv.get("workload") != workload:
+        if kv.get("workload") != workload or kv.get("kernel") != kernel:
             return None, None
         return float(kv["traffic_bytes_per_pass"]), os.path.relpath(path, ROOT)
     except (OSError, KeyError, ValueError):
@@ -284,8 +288,9 @@ def main():
     achieved = bytes_alg / (ms * 1e-3) / 1e9
     workload = (f"kooshball3d: 3D radial {a.spokes} Fibonacci spokes x {a.readout} samples, "
                 f"{C} coils, {a.image}^3 image, og 2 -> {N}^3 grid, KB width 4, L 512, complex64, DCF |k|^2")
-    traffic, traffic_src = read_traffic(a.traffic_file, workload) if not (a.exact or a.unit or a.split
-                                                                           or a.accumulate) else (None, None)
+    kernel = GRID_KERNELS.get(info["accumulate"], "unknown")
+    traffic, traffic_src = read_traffic(a.traffic_file, workload, kernel) if not (a.exact or a.unit or a.split
+                                                                                  or a.accumulate) else (None, None)
     out = {
         "metric": "gridded samples/s, recon ms (3D radial, 32 coils); learned-DCF vs iterative",
         "value": C * M / (ms * 1e-3),
@@ -297,16 +302,14 @@ def main():
         "config": {"workload": workload,
                    "samples_per_coil": M, "coils": C, "grid": N,
                    "tile": [info["tile_x"], info["tile_y"], info["tile_z"]], "slabs": info["slabs"],
-                   "accumulate": {0: "automatic (lds read-add-write)", 1: "lds float atomic add",
-                                  2: "lds read-add-write", 4: "gather (cell-sorted batches, vgpr cells)",
-                                  5: "matrix cores (16-cell blocks, weights on the fly)"}[a.accumulate],
+                   "accumulate": ("automatic: " if a.accumulate == 0 else "") + kernel,
                    "order": "exact (CPU summation order)" if a.exact else "sliced (fixed-order slice combine)",
                    "items": info["items"], "heavy_units": info["heavy_units"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes": bytes_alg,
-                     "kernel": "gridding pass: k_grid_items + k_combine (fr_grid_execute), HIP events on the "
+                     "kernel": f"gridding pass: {kernel} + k_combine (fr_grid_execute), HIP events on the "
                                "context stream"},
         "plan_ms": plan_ms, "plan_taps": info["taps"], "plan_bytes": info["plan_bytes"],
     }

@@ -1,7 +1,8 @@
-// Gridding on the matrix cores (accumulate mode FR_ACC_MFMA).
+// Gridding on the matrix cores (accumulate mode FR_ACC_MFMA; the default for
+// plans that are not exact-order).
 //
-// Same work items, records and deterministic slice combine as the LDS
-// kernels (fr_grid.hip), different arithmetic: a unit of at most 64 cells is
+// Same work items, records, taps and deterministic slice combine as the LDS
+// kernel (fr_grid.hip), different arithmetic: a unit of at most 64 cells is
 // cut into blocks of 16 cells, and its records are taken four at a time.  For
 // each block the four records' contributions are one dense product
 //
@@ -9,26 +10,25 @@
 //
 // on v_mfma_f32_16x16x4_f32, with
 //   A[i][k] = LUT weight of record k's sample at cell i (0 where the sample
-//             has no tap), computed on the fly with the reference arithmetic
-//             of fr_taps.h: the tap set {cells with sqrtf(dz^2+dy^2+dx^2) <
-//             hw} equals the reference's clamped box test (GridLut.cpp:99-
-//             136; the box bounds c +/- hw are exact or only move across
-//             cells the distance test rejects), and the LUT index is the
-//             reference's roundf((dk / hw) * (L - 1));
-//   B[k][j]  = dcf * sample component j of record k (GridLut.cpp:109).
+//             has no tap there): the plan's taps (reference tap set and LUT
+//             index, GridLut.cpp:99-136) scattered into a per-wave LDS tile;
+//   B[k][j] = dcf * sample component j of record k (GridLut.cpp:109).
 // Every product w * (dcf * s) is the reference's; only the order of the sum
 // differs (the matrix core adds four products at a time), so results agree
 // with the CPU loop to FP32 rounding (relative L2 ~1e-7), and are
-// deterministic.  No tap stream is read: a pass streams the record headers,
-// the sample rows and coordinates, and writes the grid once.
+// deterministic.
 //
 // Lane layout of v_mfma_f32_16x16x4_f32 (wave64): lane l supplies A[l % 16][l
 // / 16] and B[l / 16][l % 16]; accumulator register r of lane l holds
-// D[4 (l / 16) + r][l % 16].  So lane l works on record l / 16 of the group:
-// it loads that record's sample coordinates (for A) and its component slice
-// l % 16 (for B).  Components: with NCB component blocks per 16 lanes, lane j
-// holds components [NCB j, NCB j + NCB) of a chunk of 8 NCB coils (NCB = 4:
-// coils 2j and 2j+1; NCB = 2: coil j; NCB = 1: coil j/2, re or im).
+// D[4 (l / 16) + r][l % 16].  So lane l works on record l / 16 of a group and
+// component slice l % 16.  Components: with NCB component blocks per 16
+// lanes, lane j holds components [NCB j, NCB j + NCB) of a chunk of 8 NCB
+// coils (NCB = 4: coils 2j and 2j+1; NCB = 2: coil j; NCB = 1: coil j/2, re
+// or im).
+//
+// The accumulators leave through an LDS transpose, so that consecutive lanes
+// store consecutive cells of one coil (rows of ux cells): x-elongated units
+// (8 x 2 x 2 by default) give the longest contiguous runs.
 //
 // Non-finite sample values differ from the reference in one respect: a
 // sample's row meets every cell of the blocks it touches (with weight 0), so
@@ -176,7 +176,12 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
                                                   float2 *__restrict__ out, float2 *__restrict__ partial,
                                                   int64_t nslots, int cells_max)
 {
-    __shared__ float tile[RB * (16 * NBLK + 1)];
+    // LDS: the A tile during the record loop, then the write-out transpose
+    constexpr int CS = 16 * NBLK + 1;           // cells per component row of the transpose
+    constexpr int TILE = RB * (16 * NBLK + 1), OSM = 16 * NCB * CS;
+    __shared__ float smem[(TILE > OSM ? TILE : OSM)];
+    __shared__ uint32_t goff[16 * NBLK];        // grid offset of each unit cell
+    float *tile = smem, *osm = smem;
     const int lane = threadIdx.x;
     const int j = lane & 15;                    // A row (cell in block) / B column (component)
     const int k = lane >> 4;                    // A column / B row (record in group)
@@ -245,36 +250,37 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
             }
         }
 
-        // accumulator register r of block b: cell 16 b + 4 k + r, component slice j
+        // Write-out.  Accumulator register r of block b holds cell 16 b + 4 k
+        // + r, component NCB j + cb (coil c = re 2c, im 2c + 1).  They go to
+        // LDS as [component][cell], then consecutive lanes store consecutive
+        // cells of one coil, so each store covers whole rows of the unit (ux
+        // contiguous cells) instead of one 8-byte value in each of 32 lines.
+        __syncthreads();
+        for (int q = lane; q < ncell; q += 64) {
+            const int lx = q % ux, ly = (q / ux) % uy, lz = q / (ux * uy);
+            goff[q] = (uint32_t)(((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx));
+        }
 #pragma unroll
-        for (int b = 0; b < NBLK; ++b) {
+        for (int b = 0; b < NBLK; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int q = 16 * b + 4 * k + r;
-                // NCB = 1: lanes j (re) and j ^ 1 (im) hold one coil
-                const float other = NCB == 1 ? __shfl_xor(acc[b][0][r], 1) : 0.f;
-                if (q >= ncell) continue;
-                const int lx = q % ux, ly = (q / ux) % uy, lz = q / (ux * uy);
-                const size_t gi = ((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx);
-                auto put = [&](int cl, float re, float im) {
-                    const int c = c0 + cl;
-                    if (it.slot < 0) {
-                        out[(size_t)c * cells + gi] = make_float2(re, im);
-                    } else {
-                        const size_t slot = ((size_t)(c >> 5) * nslots + it.slot) * 32 + (c & 31);
-                        partial[slot * cells_max + q] = make_float2(re, im);
-                    }
-                };
-                if constexpr (NCB == 4) {
-                    if (v0ok) put(cl0, acc[b][0][r], acc[b][1][r]);
-                    if (v1ok) put(cl0 + 1, acc[b][2][r], acc[b][3][r]);
-                } else if constexpr (NCB == 2) {
-                    if (v0ok) put(cl0, acc[b][0][r], acc[b][1][r]);
-                } else {
-                    if ((j & 1) == 0 && v0ok) put(cl0, acc[b][0][r], other);
-                }
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) osm[(NCB * j + cb) * CS + 16 * b + 4 * k + r] = acc[b][cb][r];
+        __syncthreads();
+        const int pairs = nc * ncell;
+        for (int t = lane; t < pairs; t += 64) {
+            const int cl = t / ncell;
+            const int q = t - cl * ncell;
+            const float2 v = make_float2(osm[(2 * cl) * CS + q], osm[(2 * cl + 1) * CS + q]);
+            const int c = c0 + cl;
+            if (it.slot < 0) {
+                out[(size_t)c * cells + goff[q]] = v;
+            } else {
+                const size_t slot = ((size_t)(c >> 5) * nslots + it.slot) * 32 + (c & 31);
+                partial[slot * cells_max + q] = v;
             }
         }
+        __syncthreads();
     }
 }
 

@@ -61,8 +61,8 @@ struct fr_grids {
 #define FR_ACC_GATHER 4     // cell-sorted batches, sample rows in LDS, VGPR accumulators
 #define FR_GATHER_BATCH 32  // records per gather batch
 #define FR_GATHER_CELLS 32  // unit cells held in VGPRs by the gather kernel
-#define FR_ACC_MFMA 5       // matrix-core products of 16-cell blocks, weights computed on the fly
-#define FR_MFMA_MAX_L 4096  // LUT entries the matrix-core kernel stages in LDS
+#define FR_ACC_MFMA 5       // matrix-core products of 16-cell blocks (weights from the plan's taps)
+#define FR_MFMA_MAX_L 4096  // LUT lengths for which the plan builds exact index thresholds (degridding)
 #define FR_MFMA_WAVES_PER_CU 8    // persistent waves of the matrix-core kernel per CU (measured best of 4-32)
 
 // One batch of a gather-mode item: records [r0, r0 + nrec) and their taps

@@ -471,7 +471,11 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     fr_ctx *ctx = p->ctx;
     hipStream_t st = ctx->stream;
     auto t0 = std::chrono::steady_clock::now();
+    // automatic accumulation: the matrix cores, or LDS read-add-write for an
+    // exact-order plan (the only mode that sums in the CPU loop's order)
     int acc_mode = tune ? tune->accumulate : 0;
+    const bool exact = tune && tune->exact_order;
+    if (acc_mode == 0) acc_mode = exact ? FR_ACC_LDS_RMW : FR_ACC_MFMA;
     fr_geom geom = choose_geom(p->dim, p->N, s->C, acc_mode, tune);
     const int stride = 2 * std::min(s->C, 32) + 1;
     FR_CHECK(ctx, geom.bx >= 1 && geom.by >= 1 && geom.bz >= 1 && geom.bx <= 255 && geom.by <= 255 && geom.bz <= 255,
@@ -486,8 +490,6 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     FR_CHECK(ctx, p->accumulate != FR_ACC_MFMA || !(tune && tune->exact_order), FR_ERR_INVALID,
              "exact_order needs a sequential accumulation mode (1, 2 or 4); the matrix cores sum four products at "
              "a time");
-    FR_CHECK(ctx, p->accumulate != FR_ACC_MFMA || p->L <= FR_MFMA_MAX_L, FR_ERR_UNSUPPORTED,
-             "matrix-core accumulation stages the LUT in LDS: L %d exceeds %d", p->L, FR_MFMA_MAX_L);
     FR_CHECK(ctx, p->accumulate != FR_ACC_GATHER || geom.cells_max <= FR_GATHER_CELLS, FR_ERR_INVALID,
              "gather accumulation needs a unit box of at most %d cells (have %dx%dx%d)", FR_GATHER_CELLS, geom.bx,
              geom.by, geom.bz);
@@ -495,8 +497,7 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     // memory fault on gfx950 whose cause is not established: it is refused.
     FR_CHECK(ctx, p->accumulate != FR_ACC_REGS, FR_ERR_UNSUPPORTED,
              "register accumulation (accumulate = 3) is disabled in this build");
-    if (p->accumulate == 0) p->accumulate = FR_ACC_LDS_RMW;
-    p->exact_order = tune ? (tune->exact_order != 0) : 0;
+    p->exact_order = exact ? 1 : 0;
 
     const int N = (int)p->N;
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
@@ -860,6 +861,6 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->heavy_units = p->nheavy;
     info->max_unit_taps = p->max_unit_taps;
     info->exact_order = p->exact_order;
-    info->pad = 0;
+    info->accumulate = p->accumulate;
     return FR_OK;
 }

@@ -52,7 +52,7 @@ class fr_plan_info(C.Structure):
                 ("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
                 ("tiles", C.c_int), ("lds_bytes", C.c_int), ("plan_bytes", C.c_int64), ("build_ms", C.c_double),
                 ("items", C.c_int64), ("heavy_units", C.c_int64), ("max_unit_taps", C.c_int64),
-                ("exact_order", C.c_int), ("pad", C.c_int)]
+                ("exact_order", C.c_int), ("accumulate", C.c_int)]
 
 
 _lib = None

@@ -18,9 +18,11 @@
  *     borrowed for the duration of the call.  Calls taking device pointers
  *     are asynchronous on the context stream.
  *   - One context per host thread.
- *   - Results are deterministic: the gridding kernels use no global atomics
- *     and accumulate every grid cell in ascending sample order, the order of
- *     the reference CPU loop (ReconData/GridLut.cpp:96-145).
+ *   - Results are deterministic: the gridding kernels use no global atomics.
+ *     With exact_order every grid cell accumulates in ascending sample order,
+ *     the order of the reference CPU loop (ReconData/GridLut.cpp:96-145), and
+ *     the grid is bit-identical to it; otherwise sums are grouped (sample-
+ *     range slices, four-product matrix-core steps) in a fixed order.
  */
 #ifndef FASTRECON_HIP_H
 #define FASTRECON_HIP_H
@@ -57,7 +59,8 @@ typedef struct {
     int tile_x, tile_y, tile_z;  /* base work-unit box (one wave owns a unit) */
     int slabs;                   /* waves per workgroup                        */
     int accumulate;              /* where a unit's cells accumulate:
-                                    0: automatic (LDS read-add-write),
+                                    0: automatic (matrix cores; LDS read-add-write
+                                       for exact_order plans),
                                     1: LDS float atomic add, 2: LDS read-add-write,
                                     3: VGPRs -- refused (FR_ERR_UNSUPPORTED),
                                     4: gather -- per-batch cell-sorted taps, the
@@ -65,11 +68,10 @@ typedef struct {
                                        cell (unit box of at most 32 cells),
                                     5: matrix cores -- 16-cell blocks of a unit
                                        accumulate four records per
-                                       v_mfma_f32_16x16x4_f32 with tap weights
-                                       computed on the fly (reference tap set and
-                                       LUT index, exact); sums group four
-                                       products, so exact_order is refused;
-                                       L <= 4096; no tap stream is stored */
+                                       v_mfma_f32_16x16x4_f32, weights from the
+                                       plan's taps (reference tap set and LUT
+                                       index); sums group four products, so
+                                       exact_order is refused */
     int split_taps;              /* heavier units are split (0: automatic)      */
     int exact_order;             /* 1: split units by space only, so every cell
                                     sums in ascending sample order (bit-identical
@@ -84,7 +86,8 @@ typedef struct {
     int64_t plan_bytes;
     double build_ms;
     int64_t items, heavy_units, max_unit_taps;
-    int exact_order, pad;
+    int exact_order;
+    int accumulate;              /* the plan's resolved accumulation mode (1, 2, 4 or 5) */
 } fr_plan_info;
 
 /* ------------------------------------------------------------ host helpers */
