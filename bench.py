@@ -76,7 +76,8 @@ GRID_KERNELS = {1: "k_grid_items (LDS atomic add)", 2: "k_grid_items (LDS read-a
 
 def read_traffic(path, workload, kernel):
     """HBM bytes per gridding pass from a committed counter summary, used
-    only when it was measured on this exact workload and kernel."""
+    only when it was measured on this exact workload and kernel (kernel name
+    and unit box)."""
     try:
         kv = {}
         with open(path) as f:
@@ -288,7 +289,8 @@ def main():
     achieved = bytes_alg / (ms * 1e-3) / 1e9
     workload = (f"kooshball3d: 3D radial {a.spokes} Fibonacci spokes x {a.readout} samples, "
                 f"{C} coils, {a.image}^3 image, og 2 -> {N}^3 grid, KB width 4, L 512, complex64, DCF |k|^2")
-    kernel = GRID_KERNELS.get(info["accumulate"], "unknown")
+    kernel = (GRID_KERNELS.get(info["accumulate"], "unknown")
+              + f", unit {info['tile_x']}x{info['tile_y']}x{info['tile_z']}")
     traffic, traffic_src = read_traffic(a.traffic_file, workload, kernel) if not (a.exact or a.unit or a.split
                                                                                   or a.accumulate) else (None, None)
     out = {

@@ -87,9 +87,9 @@ static std::vector<fr_tune> default_candidates(int dim)
 {
     std::vector<fr_tune> c;
     const int shapes3[][4] = {{4, 4, 2, 2}, {4, 2, 2, 2}, {8, 4, 2, 2}, {4, 4, 2, 4},
-                              {8, 2, 2, 5}, {8, 2, 1, 5}, {4, 2, 2, 5}, {8, 1, 4, 5}};
+                              {8, 2, 1, 5}, {4, 2, 2, 5}, {8, 1, 2, 5}, {4, 4, 1, 5}};
     const int shapes2[][4] = {{8, 4, 1, 2}, {4, 4, 1, 2}, {16, 4, 1, 2}, {8, 4, 1, 4},
-                              {8, 4, 1, 5}, {16, 2, 1, 5}, {8, 2, 1, 5}, {4, 4, 1, 5}};
+                              {8, 2, 1, 5}, {16, 2, 1, 5}, {8, 4, 1, 5}, {4, 4, 1, 5}};
     const int(*shapes)[4] = dim == 3 ? shapes3 : shapes2;
     for (int i = 0; i < 8; ++i) {
         fr_tune t{};

@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace {
 
@@ -59,6 +60,7 @@ __device__ __forceinline__ float4 ld_pair(const __half2 *__restrict__ row, int c
 }
 
 constexpr int RB = 16;                       // records per batch
+constexpr int64_t XCD_RUN = FR_MFMA_XCD_RUN; // consecutive items one XCD takes in turn
 constexpr int TAP_ROW = 4 * FR_ACC_STRIDE;   // plan taps: (cell * TAP_ROW, LUT weight bits)
 // the pipeline reads headers up to 2 RB + 15 records past a live record; the
 // plan pads H by 64 records
@@ -68,18 +70,24 @@ static_assert(2 * RB + 15 < 64, "record header padding");
 // record l of the batch; tap pass p gives lane l tap u = l % TPR of record
 // p * RPP + l / TPR (a record carries at most TPR = 16 NBLK taps); rows:
 // group q gives lane (j, k) the component slice of record 4 q + k.
-template <int NBLK>
+template <int NBLK, bool CT>
 struct batch_st {
     uint32_t s;                 // lane l < RB: sample of record l
     uint32_t n, excl;           // its (padded) tap count and first tap within the batch
     float w;                    // its DCF (0 past the item's end)
-    uint2 tp[4 * NBLK];
+    typename std::conditional<CT, uint32_t, uint2>::type tp[4 * NBLK];   // compact or full taps
     float2 a[4], b[4];
 };
 
-template <int NBLK, int NCB, bool PAIR, typename TS>
-__device__ __forceinline__ uint32_t issue_batch(batch_st<NBLK> &S, uint2 h, uint32_t rb, uint32_t r1, uint32_t tbase,
-                                                const uint2 *__restrict__ T, const TS *__restrict__ data,
+// tap -> (cell within the unit, LUT weight)
+__device__ __forceinline__ int tap_cell(uint32_t t) { return (int)(t & 63u); }
+__device__ __forceinline__ int tap_cell(uint2 t) { return (int)(t.x / TAP_ROW); }
+__device__ __forceinline__ float tap_weight(uint32_t t, const float *slut) { return slut[t >> 6]; }
+__device__ __forceinline__ float tap_weight(uint2 t, const float *) { return __uint_as_float(t.y); }
+
+template <int NBLK, int NCB, bool PAIR, bool CT, typename TS, typename TT>
+__device__ __forceinline__ uint32_t issue_batch(batch_st<NBLK, CT> &S, uint2 h, uint32_t rb, uint32_t r1,
+                                                uint32_t tbase, const TT *__restrict__ T, const TS *__restrict__ data,
                                                 const float *__restrict__ dcf, uint32_t C, int ca, int cbb, int lane)
 {
     constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
@@ -120,9 +128,9 @@ __device__ __forceinline__ uint32_t issue_batch(batch_st<NBLK> &S, uint2 h, uint
     return (uint32_t)__builtin_amdgcn_readlane((int)pre, RB - 1);
 }
 
-template <int NBLK, int NCB>
-__device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st<NBLK> &S, float *tile, bool v0ok,
-                                            bool v1ok, int lane)
+template <int NBLK, int NCB, bool CT>
+__device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st<NBLK, CT> &S, float *tile,
+                                            const float *slut, bool v0ok, bool v1ok, int lane)
 {
     constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
     constexpr int ROW = 16 * NBLK + 1;          // cells (and the padding taps' row) per record column
@@ -136,8 +144,7 @@ __device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st
     for (int p = 0; p < 4 * NBLK; ++p) {
         const int rho = p * RPP + lane / TPR;
         const uint32_t nr = (uint32_t)__shfl((int)S.n, rho);
-        if ((uint32_t)(lane % TPR) < nr)
-            tile[rho * ROW + (int)(S.tp[p].x / TAP_ROW)] = __uint_as_float(S.tp[p].y);
+        if ((uint32_t)(lane % TPR) < nr) tile[rho * ROW + tap_cell(S.tp[p])] = tap_weight(S.tp[p], slut);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -168,14 +175,24 @@ __device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st
 // NBLK blocks of 16 cells; NCB component blocks (chunks of 8 NCB coils).
 // PAIR: C even, so lane j's two coils (2j, 2j+1 of a 32-coil chunk) are one
 // aligned 16-byte (8-byte for FP16) load.
-template <int NBLK, int NCB, bool PAIR, typename TS>
+// CT: the plan's taps are compact (uint16_t: cell | LUT index << 6) and the
+// weights come from the LUT staged in LDS (entry L is 0, for padding taps).
+template <int NBLK, int NCB, bool PAIR, bool CT, typename TS>
 __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ items, int64_t nitems,
                                                   const fr_unit *__restrict__ units, const uint2 *__restrict__ H,
-                                                  const uint2 *__restrict__ T, const TS *__restrict__ data,
+                                                  const void *__restrict__ Tv, const TS *__restrict__ data,
                                                   const float *__restrict__ dcf, int C, int N, size_t cells,
                                                   float2 *__restrict__ out, float2 *__restrict__ partial,
-                                                  int64_t nslots, int cells_max)
+                                                  int64_t nslots, int cells_max, const float *__restrict__ lut,
+                                                  int L, int64_t nfront)
 {
+    using TT = typename std::conditional<CT, uint16_t, uint2>::type;
+    const TT *__restrict__ T = reinterpret_cast<const TT *>(Tv);
+    __shared__ float slut[CT ? FR_CT_MAX_L + 1 : 1];
+    if constexpr (CT) {
+        for (int q = threadIdx.x; q <= L; q += 64) slut[q] = q < L ? lut[q] : 0.f;
+        __syncthreads();
+    }
     // LDS: the A tile during the record loop, then the write-out transpose
     constexpr int CS = 16 * NBLK + 1;           // cells per component row of the transpose
     constexpr int TILE = RB * (16 * NBLK + 1), OSM = 16 * NCB * CS;
@@ -196,13 +213,22 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     // inside the chunk, ca + 1 <= c0 + nc - 1 < C
     const int ca = PAIR ? c0 + min(cl0, nc - 2) : c0 + min(cl0, nc - 1), cbb = c0 + min(cl0 + 1, nc - 1);
 
-    // Persistent waves.  Round t covers items [t W, t W + W), W = gridDim.x;
-    // workgroups are dealt to the 8 XCDs round-robin, so XCD x's waves take
-    // the contiguous range x W/8 .. (x+1) W/8 - 1 of each round: neighbouring
-    // units (which share samples) run on one XCD and share its L2.
-    const int64_t W = gridDim.x;
-    const int64_t slot_in_round = (int64_t)(blockIdx.x % 8) * (W / 8) + blockIdx.x / 8;
-    for (int64_t i = slot_in_round; i < nitems; i += W) {
+    // One workgroup per item, in the order of k_grid_items (fr_grid.hip):
+    // the heaviest units' items first, one per workgroup; the rest, in
+    // Morton order, remapped so that each XCD (workgroups are dealt to the 8
+    // XCDs round-robin) takes runs of XCD_RUN consecutive items.  The hardware
+    // dispatches workgroups in order, so each XCD's running items stay a
+    // compact patch of neighbouring units that share samples in its L2 (a
+    // persistent static schedule drifts apart and measured slower).
+    const int64_t bid = blockIdx.x;
+    int64_t i0;
+    if (bid < nfront) {
+        i0 = bid;
+    } else {
+        const int64_t rr = bid - nfront, gg = rr / 8;
+        i0 = nfront + (gg / XCD_RUN) * (8 * XCD_RUN) + (bid % 8) * XCD_RUN + gg % XCD_RUN;
+    }
+    for (int64_t i = i0; i < nitems && i == i0; ++i) {
         const fr_item it = items[i];
         const fr_unit U = units[it.unit];
         const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
@@ -222,27 +248,27 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
             const uint32_t r1 = it.r1;
             uint32_t r = it.r0, tb = it.t0;
             const uint32_t hl = (uint32_t)(lane & (RB - 1));
-            batch_st<NBLK> S0, S1;
-            uint32_t T0 = issue_batch<NBLK, NCB, PAIR>(S0, H[r + hl], r, r1, tb, T, data, dcf, (uint32_t)C, ca, cbb,
+            batch_st<NBLK, CT> S0, S1;
+            uint32_t T0 = issue_batch<NBLK, NCB, PAIR, CT>(S0, H[r + hl], r, r1, tb, T, data, dcf, (uint32_t)C, ca, cbb,
                                                        lane);
             uint2 h1 = H[r + RB + hl], h3;
             uint32_t T1;
             while (true) {
                 const uint2 h2 = H[r + 2 * RB + hl];
-                T1 = issue_batch<NBLK, NCB, PAIR>(S1, h1, r + RB, r1, tb + T0, T, data, dcf, (uint32_t)C, ca, cbb,
+                T1 = issue_batch<NBLK, NCB, PAIR, CT>(S1, h1, r + RB, r1, tb + T0, T, data, dcf, (uint32_t)C, ca, cbb,
                                                   lane);
                 // keep the prefetch ahead of the compute (the scheduler would
                 // otherwise sink each load to its use and wait on it there)
                 __builtin_amdgcn_sched_barrier(0);
-                apply_batch<NBLK, NCB>(acc, S0, tile, v0ok, v1ok, lane);
+                apply_batch<NBLK, NCB, CT>(acc, S0, tile, slut, v0ok, v1ok, lane);
                 r += RB;
                 tb += T0;
                 if (r >= r1) break;
                 h3 = H[r + 2 * RB + hl];
-                T0 = issue_batch<NBLK, NCB, PAIR>(S0, h2, r + RB, r1, tb + T1, T, data, dcf, (uint32_t)C, ca, cbb,
+                T0 = issue_batch<NBLK, NCB, PAIR, CT>(S0, h2, r + RB, r1, tb + T1, T, data, dcf, (uint32_t)C, ca, cbb,
                                                   lane);
                 __builtin_amdgcn_sched_barrier(0);
-                apply_batch<NBLK, NCB>(acc, S1, tile, v0ok, v1ok, lane);
+                apply_batch<NBLK, NCB, CT>(acc, S1, tile, slut, v0ok, v1ok, lane);
                 r += RB;
                 tb += T1;
                 if (r >= r1) break;
@@ -284,20 +310,28 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     }
 }
 
-template <int NBLK, int NCB, bool PAIR, typename TS>
-fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out)
+template <int NBLK, int NCB, bool PAIR, bool CT, typename TS>
+fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out)
 {
     fr_ctx *ctx = p->ctx;
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int chunks = (s->C + 8 * NCB - 1) / (8 * NCB);
-    // persistent waves: a multiple of 8 (one share per XCD), at most one per item
-    const int64_t want = (int64_t)std::max(1, ctx->cu_count) * FR_MFMA_WAVES_PER_CU;
-    const int64_t W = std::max<int64_t>(8, std::min<int64_t>(want, (p->nitems + 7) / 8 * 8));
-    hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
-                       p->items, p->nitems, p->units, p->hdrs, p->taps, (const TS *)s->data, s->dcf, s->C, (int)p->N,
-                       cells, out, (float2 *)p->partial, p->nslots, p->geom.cells_max);
+    const int64_t light = p->nitems - p->nitems_front;
+    const int64_t span = 8 * XCD_RUN;
+    const int64_t W = p->nitems_front + span * ((light + span - 1) / span);
+    hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, CT, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
+                       p->items, p->nitems, p->units, p->hdrs, (const void *)p->taps, (const TS *)s->data, s->dcf,
+                       s->C, (int)p->N, cells, out, (float2 *)p->partial, p->nslots, p->geom.cells_max, p->lut,
+                       p->L, p->nitems_front);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
+}
+
+template <int NBLK, int NCB, bool PAIR, typename TS>
+fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out)
+{
+    return p->compact_taps ? launch_mfma_ct<NBLK, NCB, PAIR, true, TS>(p, s, out)
+                           : launch_mfma_ct<NBLK, NCB, PAIR, false, TS>(p, s, out);
 }
 
 template <int NBLK, typename TS>

@@ -63,7 +63,10 @@ struct fr_grids {
 #define FR_GATHER_CELLS 32  // unit cells held in VGPRs by the gather kernel
 #define FR_ACC_MFMA 5       // matrix-core products of 16-cell blocks (weights from the plan's taps)
 #define FR_MFMA_MAX_L 4096  // LUT lengths for which the plan builds exact index thresholds (degridding)
-#define FR_MFMA_WAVES_PER_CU 8    // persistent waves of the matrix-core kernel per CU (measured best of 4-32)
+#define FR_CT_MAX_L 1023    // longest LUT for compact 2-byte taps (10-bit index, index L = zero weight)
+#ifndef FR_MFMA_XCD_RUN
+#define FR_MFMA_XCD_RUN 128       // items per XCD run in the matrix-core kernel's launch order
+#endif
 
 // One batch of a gather-mode item: records [r0, r0 + nrec) and their taps
 // T[toff ..) sorted by cell; cnt byte k = tap count of cell k (a multiple
@@ -128,7 +131,9 @@ struct fr_plan {
     fr_unit *units = nullptr;          // device, heaviest first
     int64_t nunits = 0;
     uint2 *hdrs = nullptr;             // records: (sample, ntaps)
-    uint2 *taps = nullptr;             // taps: (LDS byte offset of the cell's row, LUT weight bits)
+    uint2 *taps = nullptr;             // taps: (LDS byte offset of the cell's row, LUT weight bits), or
+                                       // uint16_t (cell | LUT index << 6) when compact_taps
+    bool compact_taps = false;
     int64_t nrecords = 0, ntaps = 0;
     int64_t max_unit_taps = 0;
     int exact_order = 0;               // 1: no sample slicing (bitwise CPU order)

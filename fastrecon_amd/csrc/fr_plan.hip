@@ -165,7 +165,8 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
                                const int32_t *__restrict__ cell_unit, const fr_unit *__restrict__ units,
                                const uint32_t *__restrict__ keys, const uint64_t *__restrict__ vals,
                                const uint64_t *__restrict__ toff, int64_t R, const float *__restrict__ lut,
-                               uint32_t tap_scale, uint32_t pad_x, uint2 *__restrict__ H, uint2 *__restrict__ T)
+                               uint32_t tap_scale, uint32_t pad_x, uint2 *__restrict__ H, uint2 *__restrict__ T,
+                               uint16_t *__restrict__ T16)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
@@ -174,11 +175,25 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
     const uint32_t n = (uint32_t)(vals[r] & 0xffffffffu);
     const uint32_t np = padded_taps(n);
     H[r] = make_uint2((uint32_t)s, np);
-    if (!T) return;                             // records only
+    if (!T && !T16) return;                     // records only
     fr_box b;
     fr_sample_box(g.dim, g.N, traj, M, s, g.hw, b, g.frames);
-    uint2 *w = T + toff[r];
     uint32_t i = 0;
+    if (T16) {
+        // compact taps: cell | LUT index << 6; padding: cell pad_x, index L (a zero weight)
+        uint16_t *w = T16 + toff[r];
+        for_each_unit(b, g.N, cell_unit, units, [&](int u, const fr_unit &U, const int *lo, const int *hi) {
+            if ((uint32_t)u != key) return;
+            const int ux = U.ext & 255, uy = (U.ext >> 8) & 255;
+            fr_box_taps(b, lo, hi, g.hw, g.L, [&](int x, int y, int z, int ki) {
+                const uint32_t cell = (uint32_t)(((z - U.oz) * uy + (y - U.oy)) * ux + (x - U.ox));
+                w[i++] = (uint16_t)(cell | ((uint32_t)ki << 6));
+            });
+        });
+        for (; i < np; ++i) w[i] = (uint16_t)(pad_x | ((uint32_t)g.L << 6));
+        return;
+    }
+    uint2 *w = T + toff[r];
     for_each_unit(b, g.N, cell_unit, units, [&](int u, const fr_unit &U, const int *lo, const int *hi) {
         if ((uint32_t)u != key) return;
         const int ux = U.ext & 255, uy = (U.ext >> 8) & 255;
@@ -389,8 +404,9 @@ int floor_pow2(int v)
 // MI355X 16-32 cells per unit measured best for the 32-coil kooshball.
 //
 // The matrix-core mode (FR_ACC_MFMA) pays per (record, 16-cell block) and
-// writes rows of ux cells per coil: its base unit is 8 x 2 x 2 in 3D, 8 x 4 in
-// 2D (x-elongated shapes measured fastest on MI355X).
+// writes rows of ux cells per coil: its base unit is one x-elongated block,
+// 8 x 2 x 1 in 3D and 8 x 2 in 2D (of the 16-64-cell shapes measured on
+// MI355X, 8 x 2 x 1 was fastest for the 32-coil 256^3 kooshball).
 static fr_geom choose_geom(int dim, unsigned N, int coils, int accumulate, const fr_tune *tune)
 {
     (void)coils;
@@ -399,8 +415,8 @@ static fr_geom choose_geom(int dim, unsigned N, int coils, int accumulate, const
     g.N = (int)N;
     if (accumulate == FR_ACC_MFMA) {
         g.bx = 8;
-        g.by = dim == 3 ? 2 : 4;
-        g.bz = dim == 3 ? 2 : 1;
+        g.by = 2;
+        g.bz = 1;
     } else {
         g.bx = dim == 3 ? 4 : 8;
         g.by = 4;
@@ -662,13 +678,16 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     FR_CHECK(ctx, TT + 1024 < (uint64_t(1) << 32), FR_ERR_UNSUPPORTED, "plan has %llu taps (max 2^32 - 1024)",
              (unsigned long long)TT);
 
-    // 5. unit record/tap ranges, then the records themselves
-    const bool write_taps = true;
+    // 5. unit record/tap ranges, then the records themselves.  Matrix-core
+    // plans store compact 2-byte taps (cell, LUT index) when they fit: cells
+    // < 64 and L <= FR_CT_MAX_L (index L is the padding taps' zero weight).
+    p->compact_taps = p->accumulate == FR_ACC_MFMA && geom.cells_max <= 63 && p->L <= FR_CT_MAX_L;
+    const size_t tap_bytes = p->compact_taps ? sizeof(uint16_t) : sizeof(uint2);
     const uint64_t tap_alloc = TT + 576;
     if ((e = fr_alloc(ctx, (void **)&p->hdrs, sizeof(uint2) * (R + 64)))) return e;
     // padded so a wave may stage 512 taps from any 64-aligned chunk base
-    if ((e = fr_alloc(ctx, (void **)&p->taps, sizeof(uint2) * tap_alloc))) return e;
-    FR_HIP(ctx, hipMemsetAsync(p->taps, 0, sizeof(uint2) * tap_alloc, st));
+    if ((e = fr_alloc(ctx, (void **)&p->taps, tap_bytes * tap_alloc))) return e;
+    FR_HIP(ctx, hipMemsetAsync(p->taps, 0, tap_bytes * tap_alloc, st));
     FR_HIP(ctx, hipMemsetAsync(p->hdrs, 0, sizeof(uint2) * (R + 64), st));
     // tap cell field: the byte offset of the cell's accumulator row; padding
     // taps go to the spare row cells_max with weight 0
@@ -676,14 +695,17 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     // below re-sorts them)
     const bool gather = p->accumulate == FR_ACC_GATHER;
     const uint32_t tap_scale = gather ? 1u : 4u * FR_ACC_STRIDE;
-    const uint32_t pad_x = gather ? 0xffffu : (uint32_t)geom.cells_max * 4u * FR_ACC_STRIDE;
+    const uint32_t pad_x = gather ? 0xffffu
+                                  : (p->compact_taps ? (uint32_t)geom.cells_max
+                                                     : (uint32_t)geom.cells_max * 4u * FR_ACC_STRIDE);
     if (R) {
         hipLaunchKernelGGL(k_unit_ranges, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, (int64_t)R,
                            p->units);
         hipLaunchKernelGGL(k_unit_counts, dim3(nblk(nunits, 256)), dim3(256), 0, st, p->units, nunits, toff.p);
         hipLaunchKernelGGL(k_fill_records, dim3(nblk(R, 128)), dim3(128), 0, st, s->traj, M, g, cu_d.p, p->units,
                            keys_out.p, vals_out.p, toff.p, (int64_t)R, p->lut, tap_scale, pad_x, p->hdrs,
-                           write_taps ? p->taps : nullptr);
+                           p->compact_taps ? nullptr : p->taps,
+                           p->compact_taps ? reinterpret_cast<uint16_t *>(p->taps) : nullptr);
     }
     FR_HIP(ctx, hipGetLastError());
 
@@ -855,7 +877,9 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->slabs = p->geom.waves;
     info->tiles = (int)std::min<int64_t>(p->nunits, INT32_MAX);
     info->lds_bytes = 0;
-    info->plan_bytes = (int64_t)sizeof(uint2) * (p->nrecords + p->ntaps) + (int64_t)sizeof(fr_unit) * p->nunits;
+    info->plan_bytes = (int64_t)sizeof(uint2) * p->nrecords +
+                       (int64_t)(p->compact_taps ? sizeof(uint16_t) : sizeof(uint2)) * p->ntaps +
+                       (int64_t)sizeof(fr_unit) * p->nunits;
     info->build_ms = p->build_ms;
     info->items = p->nitems;
     info->heavy_units = p->nheavy;
