@@ -13,6 +13,7 @@
 #include "common.h"
 
 struct fr_grids;
+struct fr_fft;
 
 template<typename T>
 class ImageData
@@ -62,6 +63,12 @@ public:
     bool shiftPending() const { return m_shift_pending; }
     void setShiftPending(bool pending) { m_shift_pending = pending; }
     void materializeShift();
+    // A pending transform (ImageFilter::fftExecute): the device data is not
+    // yet transformed; SOS fuses it with the shift and crop, every other
+    // reader runs it first (before any pending shift).
+    fr_fft *fftPending() const { return m_fft_pending.get(); }
+    void setFftPending(std::shared_ptr<fr_fft> fft) { m_fft_pending = std::move(fft); }
+    void materializeFft();
 
 protected:
     virtual void copy(const ImageData<T> &imageData);
@@ -81,6 +88,7 @@ private:
     mutable bool m_host_valid = true;
     bool m_dev_valid = false;
     bool m_shift_pending = false;
+    std::shared_ptr<fr_fft> m_fft_pending;
 };
 
 #endif // IMAGEDATA_H

@@ -1,9 +1,10 @@
 // Image-stage operations (reference ImageFilter/ImageFilter.h:9-38).
 //
 // Same interface as the reference.  Every operation runs on the device
-// grids of the ImageData: fftExecute is one batched rocFFT over all
-// channels, fftShift is folded into the next reader, SOS is one fused
-// crop + sum-of-squares kernel.
+// grids of the ImageData: fftExecute is one batched transform over all
+// channels, deferred until the next reader; fftShift is folded into the next
+// reader; SOS after fftExecute runs cropped transforms that keep only the
+// reconstructed region and sum the squares over channels in their last pass.
 #ifndef IMAGEFILTER_H
 #define IMAGEFILTER_H
 
@@ -46,7 +47,7 @@ protected:
     virtual void fftShift2(ComplexVector<T> *data);
     virtual void fftShift3(ComplexVector<T> *data);
 
-    fr_fft *m_fft = nullptr;
+    std::shared_ptr<fr_fft> m_fft;
 
 private:
     ImageData<T> &m_associatedData;

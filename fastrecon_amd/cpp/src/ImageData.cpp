@@ -89,8 +89,17 @@ ComplexVector<T> *ImageData<T>::getChannelImage(int channel)
 }
 
 template<typename T>
+void ImageData<T>::materializeFft()
+{
+    if (m_fft_pending && m_dev && m_dev_valid)
+        check(fr_fft_execute(m_fft_pending.get(), m_dev.get()), "FFT");
+    m_fft_pending.reset();
+}
+
+template<typename T>
 void ImageData<T>::materializeShift()
 {
+    materializeFft();
     if (m_shift_pending && m_dev && m_dev_valid) {
         check(fr_fftshift(ctx(), m_dev.get()), "fftShift");
         m_shift_pending = false;
@@ -117,6 +126,7 @@ void ImageData<T>::dropDevice()
     m_dev.reset();
     m_dev_valid = false;
     m_shift_pending = false;
+    m_fft_pending.reset();
 }
 
 template<typename T>
@@ -135,6 +145,7 @@ fr_grids *ImageData<T>::deviceGrids()
             check(fr_grids_upload(g, c, m_data_multichannel[c].data()), "upload channel");
         m_dev_valid = true;
         m_shift_pending = false;
+        m_fft_pending.reset();
     }
     m_host_valid = false;
     return m_dev.get();
@@ -157,6 +168,7 @@ void ImageData<T>::adoptDeviceGrids(fr_grids *grids, int channels)
     m_dev = std::shared_ptr<fr_grids>(grids, fr_grids_destroy);
     m_dev_valid = true;
     m_shift_pending = false;
+    m_fft_pending.reset();
     m_channels = channels;
     m_data_multichannel.clear();
     m_host_valid = false;
@@ -177,6 +189,7 @@ void ImageData<T>::copy(const ImageData<T> &imageData)
         m_dev = std::shared_ptr<fr_grids>(g, fr_grids_destroy);
         m_dev_valid = true;
         m_shift_pending = imageData.m_shift_pending;
+        m_fft_pending = imageData.m_fft_pending;
         m_data_multichannel.clear();
         m_host_valid = false;
     } else {
@@ -199,6 +212,7 @@ void ImageData<T>::move(ImageData<T> &imageData)
     m_host_valid = imageData.m_host_valid;
     m_dev_valid = imageData.m_dev_valid;
     m_shift_pending = imageData.m_shift_pending;
+    m_fft_pending = std::move(imageData.m_fft_pending);
     imageData.m_data_multichannel.clear();
     imageData.m_dev.reset();
     imageData.m_dim = 0;
@@ -207,6 +221,7 @@ void ImageData<T>::move(ImageData<T> &imageData)
     imageData.m_host_valid = true;
     imageData.m_dev_valid = false;
     imageData.m_shift_pending = false;
+    imageData.m_fft_pending.reset();
     std::cout << "-- ImageData: move --" << std::endl;      // ImageData.cpp:110
 }
 

@@ -16,11 +16,7 @@ std::shared_ptr<ImageFilter<T>> ImageFilter<T>::Create(ImageData<T> &imageData)
 }
 
 template<typename T>
-ImageFilter<T>::~ImageFilter()
-{
-    if (m_fft != nullptr)
-        fr_fft_destroy(m_fft);
-}
+ImageFilter<T>::~ImageFilter() = default;
 
 template<typename T>
 ImageFilter<T>::ImageFilter(ImageData<T> &imageData)
@@ -99,11 +95,18 @@ void ImageFilter<T>::SOS(ImageSize reconSize, const float *deapodDevice)
     }
     ImageData<T> img(d.dim(), reconSize);
     const std::size_t n = img.dataSize();
-    const fr_grids *g = d.deviceGrids();
+    fr_grids *g = d.deviceGrids();
     float *out = nullptr;
     check(fr_malloc(ctx(), sizeof(float) * n, (void **)&out), "allocate image");
-    check(fr_sos(ctx(), g, reconSize.x, reconSize.y, reconSize.z, d.shiftPending() ? 1 : 0, deapodDevice, out),
-          "SOS");
+    if (d.fftPending() != nullptr) {
+        // transform + shift + crop + SOS in one call; the grids are consumed
+        check(fr_fft_sos(d.fftPending(), g, reconSize.x, reconSize.y, reconSize.z, d.shiftPending() ? 1 : 0,
+                         deapodDevice, out), "FFT + SOS");
+        d.setFftPending(nullptr);
+    } else {
+        check(fr_sos(ctx(), g, reconSize.x, reconSize.y, reconSize.z, d.shiftPending() ? 1 : 0, deapodDevice, out),
+              "SOS");
+    }
     std::vector<float> mag(n);
     check(fr_memcpy_d2h(ctx(), mag.data(), out, sizeof(float) * n), "download image");
     fr_free(ctx(), out);
@@ -126,6 +129,8 @@ void ImageFilter<T>::SOS(const ImageData<T> &map, ImageSize reconSize)
     }
     const fr_grids *g = d.deviceGrids();
     const fr_grids *mg = m.deviceGrids();
+    d.materializeFft();
+    m.materializeFft();
     if (d.shiftPending() != m.shiftPending()) {
         d.materializeShift();
         m.materializeShift();
@@ -148,29 +153,29 @@ void ImageFilter<T>::SOS(const ImageData<T> &map, ImageSize reconSize)
 template<typename T>
 void ImageFilter<T>::fftPlan(int sign)
 {
-    if (m_fft != nullptr) {
-        fr_fft_destroy(m_fft);
-        m_fft = nullptr;
-    }
+    m_fft.reset();
     auto &d = m_associatedData;
     if (d.imageSize().x != d.imageSize().y || (d.dim() == 3 && d.imageSize().z != d.imageSize().x)) {
         std::cerr << "Error: FFT wrong image size" << std::endl;
         exit(1);
     }
     m_sign = sign >= 0 ? +1 : -1;
-    check(fr_fft_create(ctx(), d.dim(), d.imageSize().x, d.channels(), m_sign, &m_fft), "FFT plan");
+    fr_fft *f = nullptr;
+    check(fr_fft_create(ctx(), d.dim(), d.imageSize().x, d.channels(), m_sign, &f), "FFT plan");
+    m_fft = std::shared_ptr<fr_fft>(f, fr_fft_destroy);
 }
 
 // ImageFilter.cpp:222-229, FFT.cpp:54-91, cuFFT.cpp:24-37: in place,
-// unnormalised, all channels in one execution.
+// unnormalised, all channels in one execution.  Recorded as pending: the
+// next reader runs it (SOS fuses it with the crop).
 template<typename T>
 void ImageFilter<T>::fftExecute()
 {
     if (m_fft == nullptr)
         fftPlan(m_sign);
-    fr_grids *g = m_associatedData.deviceGrids();
+    m_associatedData.deviceGrids();
     m_associatedData.materializeShift();
-    check(fr_fft_execute(m_fft, g), "FFT");
+    m_associatedData.setFftPending(m_fft);
 }
 
 // ImageFilter.cpp:231-251 (kept for subclasses; fftShift() itself runs on

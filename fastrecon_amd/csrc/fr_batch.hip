@@ -1,5 +1,6 @@
 // Launch-level machinery on top of the kernels: one reconstruction pass
-// (gridding, batched FFT, fused shift/crop/SOS) captured into a HIP graph and
+// (gridding, then the cropped transforms with the coil sum of squares in their
+// last pass) captured into a HIP graph and
 // replayed, and an autotuner that times candidate unit shapes / slice sizes
 // on the actual trajectory and keeps the fastest plan settings.
 //
@@ -18,6 +19,17 @@ struct fr_graph {
     hipGraphExec_t exec = nullptr;
 };
 
+// FFT only, or FFT + shift + crop + SOS through the cropped transforms
+// (fr_fft_sos: the grids are consumed, which a replayed pass does not mind
+// since gridding rewrites them).
+static fr_status image_stage(fr_fft *f, fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz, int apply_shift,
+                             const float *deapod_dev, float *out_dev)
+{
+    if (!f) return FR_OK;
+    if (!out_dev) return fr_fft_execute(f, g);
+    return fr_fft_sos(f, g, rcx, rcy, rcz, apply_shift, deapod_dev, out_dev);
+}
+
 extern "C" fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *g, fr_fft *f, unsigned rcx,
                                      unsigned rcy, unsigned rcz, int apply_shift, const float *deapod_dev,
                                      float *out_dev, fr_graph **out)
@@ -30,8 +42,7 @@ extern "C" fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *
     // partials) and validates the arguments, so the captured pass only
     // enqueues kernels.
     fr_status st = fr_grid_execute(p, const_cast<fr_samples *>(s), g);
-    if (!st && f) st = fr_fft_execute(f, g);
-    if (!st && out_dev) st = fr_sos(ctx, g, rcx, rcy, rcz, apply_shift, deapod_dev, out_dev);
+    if (!st) st = image_stage(f, g, rcx, rcy, rcz, apply_shift, deapod_dev, out_dev);
     if (st) return st;
     FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
 
@@ -44,8 +55,7 @@ extern "C" fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *
         return fr_fail(ctx, FR_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     }
     st = fr_grid_execute(p, const_cast<fr_samples *>(s), g);
-    if (!st && f) st = fr_fft_execute(f, g);
-    if (!st && out_dev) st = fr_sos(ctx, g, rcx, rcy, rcz, apply_shift, deapod_dev, out_dev);
+    if (!st) st = image_stage(f, g, rcx, rcy, rcz, apply_shift, deapod_dev, out_dev);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->stream, &graph);
     if (st || e != hipSuccess) {

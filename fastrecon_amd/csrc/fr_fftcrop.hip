@@ -1,0 +1,305 @@
+// Cropped FFT + fftshift + sum-of-squares in three (3D) or two (2D) passes.
+//
+// The reconstruction needs, per coil, the unnormalised sign +1 FFT of the
+// grid (ImageFilter/FFT.cpp:40-51,82-84), fftshifted (ImageFilter.cpp:231-
+// 280), cropped to rc at offset (N - rc)/2 and combined as sqrt(sum_c |v_c|^2)
+// (ImageFilter.cpp:159-209, Main.cpp:175-181).  Only rc of the N outputs of
+// each 1D transform reach the crop, so each pass transforms rows along one
+// axis, keeps those outputs and writes them transposed for the next axis:
+//
+//   3D  G[c][z][y][x]  -x->  B1[c][z][kx][y]  -y->  B2[c][ky][kx][z]  -z->  sum over c of |.|^2
+//   2D  G[c][f][y][x]  -x->  B1[c][f][kx][y]  -y->  sum over c of |.|^2  (f = frame)
+//
+// k indices are crop positions (0 .. rc-1, already fftshifted).  Each pass
+// reads its input once and writes half (rc = N/2) of it; the last pass sums
+// the coils in order, fl(re*re) + fl(im*im) per coil, then sqrtf and the
+// optional deapodisation divide: the arithmetic of k_sos (fr_image.hip).
+//
+// Row transform: N-point Stockham radix-4 (N = 4^m, or 2 * 4^m with a final
+// radix-2 step), one wave per row, twiddles exp(+2 pi i k / N) from an FP64
+// table rounded to FP32.  The result differs from rocFFT's in rounding
+// (relative ~1e-7), well inside the 1e-5 image bar.
+#include "fr_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int TR = 16;          // rows per workgroup tile
+constexpr int WGT = 256;        // threads per workgroup (4 waves)
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(__fsub_rn(__fmul_rn(a.x, b.x), __fmul_rn(a.y, b.y)),
+                       __fadd_rn(__fmul_rn(a.x, b.y), __fmul_rn(a.y, b.x)));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y)); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y)); }
+
+// Natural-order N-point transform of the row at `in` (unit stride).  After
+// the call, lane l holds output k = l + 64 q in v[q] (q = 0 .. N/64 - 1).
+// buf: 2N float2 of this wave's LDS; tw: N twiddles exp(s 2 pi i k / N).
+template <int N>
+__device__ __forceinline__ void fft_row(const float2 *__restrict__ in, float2 (&v)[N / 64], float2 *buf,
+                                        const float2 *tw, int lane, float s)
+{
+    constexpr int Q = N / 64;                    // values per lane
+    constexpr int B4 = N / 4 / 64;               // radix-4 butterflies per lane
+    float2 *src = buf, *dst = buf + N;
+    // iteration with Ns = 1 reads the row from global memory
+    int Ns = 1;
+    bool first = true;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        if (Ns * 4 > N) break;
+#pragma unroll
+        for (int b = 0; b < B4; ++b) {
+            const int j = lane + 64 * b;
+            float2 x[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x[r] = first ? in[j + r * (N / 4)] : src[j + r * (N / 4)];
+            const int jm = j % Ns;
+            const int t = jm * (N / (4 * Ns));
+#pragma unroll
+            for (int r = 1; r < 4; ++r) x[r] = cmul(x[r], tw[(t * r) & (N - 1)]);
+            const float2 a = cadd(x[0], x[2]), bb = csub(x[0], x[2]), c = cadd(x[1], x[3]);
+            const float2 dd = csub(x[1], x[3]);
+            const float2 d = make_float2(-s * dd.y, s * dd.x);          // (x1 - x3) * (s i)
+            const int idx = (j / Ns) * Ns * 4 + jm;
+            dst[idx] = cadd(a, c);
+            dst[idx + Ns] = cadd(bb, d);
+            dst[idx + 2 * Ns] = csub(a, c);
+            dst[idx + 3 * Ns] = csub(bb, d);
+        }
+        first = false;
+        float2 *tmp = src; src = dst; dst = tmp;
+        Ns *= 4;
+    }
+    if (Ns < N) {
+        // N = 2 * 4^m: one radix-2 iteration with Ns = N / 2 (output index = j)
+#pragma unroll
+        for (int b = 0; b < N / 2 / 64; ++b) {
+            const int j = lane + 64 * b;
+            const float2 x0 = src[j];
+            const float2 x1 = cmul(src[j + N / 2], tw[j]);
+            v[b] = cadd(x0, x1);
+            v[b + N / 128] = csub(x0, x1);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] = src[lane + 64 * q];
+    }
+}
+
+// crop position of output k (or -1): shifted crop k' = (k - N/2 - x0) mod N
+__device__ __forceinline__ int crop_pos(int k, int N, int rc, int shift)
+{
+    const int x0 = (N - rc) / 2;
+    int kp = shift ? k - N / 2 - x0 : k - x0;
+    kp = kp < 0 ? kp + N : kp;
+    kp = kp >= N ? kp - N : kp;
+    return kp < rc ? kp : -1;
+}
+
+// Pass over tiles of TR rows: row t of tile T starts at in + rowA(T) + t *
+// rowStride; kept output k' of row t goes to out[outA(T) + k' * outK + t].
+// Layout parameters: tile T = (a, b, c0) over dims (na, nb, nt = rows / TR).
+struct pass_desc {
+    int64_t na, nb, nt;                          // tile grid
+    int64_t inA, inB, inT, rowStride;            // in offset = a inA + b inB + (t0 + t) rowStride (t0 = tt * TR)
+    int64_t outA, outB, outT, outK;              // out offset = a outA + b outB + tt*TR*outT + k' outK + t outT
+};
+
+template <int N>
+__global__ void __launch_bounds__(WGT) k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                  const float2 *__restrict__ gtw, pass_desc d, int rc, int shift,
+                                                  float s)
+{
+    __shared__ float2 tw[N];
+    __shared__ float2 fbuf[4][2 * N];
+    extern __shared__ float2 otile[];           // [rc][TR]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < N; i += WGT) tw[i] = gtw[i];
+    __syncthreads();
+    const int64_t T = blockIdx.x;
+    const int64_t tt = T % d.nt, b = (T / d.nt) % d.nb, a = T / (d.nt * d.nb);
+    const float2 *base = in + a * d.inA + b * d.inB + tt * TR * d.rowStride;
+#pragma unroll 1
+    for (int t = w; t < TR; t += 4) {
+        float2 v[N / 64];
+        fft_row<N>(base + (int64_t)t * d.rowStride, v, fbuf[w], tw, lane, s);
+#pragma unroll
+        for (int q = 0; q < N / 64; ++q) {
+            const int kp = crop_pos(lane + 64 * q, N, rc, shift);
+            if (kp >= 0) otile[kp * TR + t] = v[q];
+        }
+    }
+    __syncthreads();
+    float2 *ob = out + a * d.outA + b * d.outB + tt * TR * d.outT;
+    for (int i = threadIdx.x; i < rc * TR; i += WGT) {
+        const int kp = i / TR, t = i % TR;
+        ob[(int64_t)kp * d.outK + (int64_t)t * d.outT] = otile[i];
+    }
+}
+
+// Last pass with the coil sum: tile (a = outer slice, tt = TR rows), coils
+// c = 0 .. C-1 in order; row t of coil c at in + c inC + a inA + (tt TR + t)
+// rowStride.  acc[k'][t] += fl(re^2) + fl(im^2); then sqrt, deapod, store to
+// img[a outA + k' outK + tt TR + t].
+template <int N>
+__global__ void __launch_bounds__(WGT) k_fft_sos(const float2 *__restrict__ in, float *__restrict__ img,
+                                                 const float2 *__restrict__ gtw, int64_t na, int64_t nt, int C,
+                                                 int64_t inC, int64_t inA, int64_t rowStride, int64_t outA,
+                                                 int64_t outK, int rc, int shift, float s,
+                                                 const float *__restrict__ deapod, int64_t dmod)
+{
+    __shared__ float2 tw[N];
+    __shared__ float2 fbuf[4][2 * N];
+    extern __shared__ float acc[];               // [rc][TR]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < N; i += WGT) tw[i] = gtw[i];
+    for (int i = threadIdx.x; i < rc * TR; i += WGT) acc[i] = 0.f;
+    __syncthreads();
+    const int64_t T = blockIdx.x;
+    const int64_t tt = T % nt, a = T / nt;
+#pragma unroll 1
+    for (int c = 0; c < C; ++c) {
+        const float2 *base = in + c * inC + a * inA + tt * TR * rowStride;
+#pragma unroll 1
+        for (int t = w; t < TR; t += 4) {
+            float2 v[N / 64];
+            fft_row<N>(base + (int64_t)t * rowStride, v, fbuf[w], tw, lane, s);
+#pragma unroll
+            for (int q = 0; q < N / 64; ++q) {
+                const int kp = crop_pos(lane + 64 * q, N, rc, shift);
+                if (kp >= 0) {
+                    float &x = acc[kp * TR + t];
+                    x = __fadd_rn(x, __fadd_rn(__fmul_rn(v[q].x, v[q].x), __fmul_rn(v[q].y, v[q].y)));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < rc * TR; i += WGT) {
+        const int kp = i / TR, t = i % TR;
+        const int64_t o = a * outA + (int64_t)kp * outK + tt * TR + t;
+        float m = sqrtf(acc[i]);
+        if (deapod) m = __fdiv_rn(m, deapod[o % dmod]);
+        img[o] = m;
+    }
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <int N>
+fr_status run_passes(fr_fft *f, const fr_grids *g, int rcx, int rcy, int rcz, int shift, const float *deapod,
+                     float *out)
+{
+    fr_ctx *ctx = f->ctx;
+    const int C = g->C;
+    const float s = (float)f->sign;
+    const size_t lds_row = sizeof(float2) * 16;
+    if (g->dim == 3) {
+        // B1[c][z][kx][y], B2[c][ky][kx][z]
+        const int64_t n1 = (int64_t)C * N * rcx * N, n2 = (int64_t)C * rcy * rcx * N;
+        fr_status e = fr_fft_workspace(f, sizeof(float2) * (size_t)(n1 + n2));
+        if (e) return e;
+        float2 *B1 = (float2 *)f->crop_ws, *B2 = B1 + n1;
+        // x: tiles (c, z, y-block); rows along x; out B1
+        pass_desc d1{C, N, N / TR, (int64_t)N * N * N, (int64_t)N * N, 0, N,
+                     (int64_t)N * rcx * N, (int64_t)rcx * N, 1, N};
+        hipLaunchKernelGGL(k_fft_pass<N>, dim3((unsigned)(C * N * (N / TR))), dim3(WGT), lds_row * rcx, ctx->stream,
+                           g->data, B1, f->tw, d1, rcx, shift, s);
+        // y: tiles (c, kx, z-block); row (c, z, kx) of B1; out B2[c][ky][kx][z]
+        pass_desc d2{C, rcx, N / TR, (int64_t)N * rcx * N, N, 0, (int64_t)rcx * N,
+                     (int64_t)rcy * rcx * N, N, 1, (int64_t)rcx * N};
+        hipLaunchKernelGGL(k_fft_pass<N>, dim3((unsigned)(C * rcx * (N / TR))), dim3(WGT), lds_row * rcy,
+                           ctx->stream, B1, B2, f->tw, d2, rcy, shift, s);
+        // z + coil sum: tiles (ky, kx-block); row (c, ky, kx) of B2; img[kz][ky][kx]
+        hipLaunchKernelGGL(k_fft_sos<N>, dim3((unsigned)(rcy * (rcx / TR))), dim3(WGT), sizeof(float) * 16 * rcz,
+                           ctx->stream, B2, out, f->tw, (int64_t)rcy, (int64_t)(rcx / TR), C,
+                           (int64_t)rcy * rcx * N, (int64_t)rcx * N, (int64_t)N, (int64_t)rcx,
+                           (int64_t)rcy * rcx, rcz, shift, s, deapod, (int64_t)rcz * rcy * rcx);
+    } else {
+        const int F = std::max(g->frames, 1);
+        const int64_t n1 = (int64_t)C * F * rcx * N;
+        fr_status e = fr_fft_workspace(f, sizeof(float2) * (size_t)n1);
+        if (e) return e;
+        float2 *B1 = (float2 *)f->crop_ws;
+        // x: tiles (c, f, y-block); out B1[c][f][kx][y]
+        pass_desc d1{C, F, N / TR, (int64_t)F * N * N, (int64_t)N * N, 0, N,
+                     (int64_t)F * rcx * N, (int64_t)rcx * N, 1, N};
+        hipLaunchKernelGGL(k_fft_pass<N>, dim3((unsigned)(C * F * (N / TR))), dim3(WGT), lds_row * rcx, ctx->stream,
+                           g->data, B1, f->tw, d1, rcx, shift, s);
+        // y + coil sum: tiles (f, kx-block); row (c, f, kx) of B1; img[f][ky][kx]
+        // (the deapodisation map is one rcy x rcx image shared by the frames)
+        hipLaunchKernelGGL(k_fft_sos<N>, dim3((unsigned)(F * (rcx / TR))), dim3(WGT), sizeof(float) * 16 * rcy,
+                           ctx->stream, B1, out, f->tw, (int64_t)F, (int64_t)(rcx / TR), C, (int64_t)F * rcx * N,
+                           (int64_t)rcx * N, (int64_t)N, (int64_t)rcy * rcx, (int64_t)rcx, rcy, shift, s, deapod,
+                           (int64_t)rcy * rcx);
+    }
+    FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+}  // namespace
+
+fr_status fr_fft_workspace(fr_fft *f, size_t bytes)
+{
+    if (bytes <= f->crop_ws_bytes) return FR_OK;
+    FR_HIP(f->ctx, hipStreamSynchronize(f->ctx->stream));
+    hipFree(f->crop_ws);
+    f->crop_ws = nullptr;
+    f->crop_ws_bytes = 0;
+    fr_status e = fr_alloc(f->ctx, &f->crop_ws, bytes);
+    if (e) return e;
+    f->crop_ws_bytes = bytes;
+    return FR_OK;
+}
+
+bool fr_fft_sos_fused(const fr_fft *f, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz)
+{
+    (void)rcz;
+    return (f->N == 256 || f->N == 512) && g->N == f->N && rcx % 16 == 0 && rcx >= 16 && rcy >= 1 &&
+           (g->dim == 2 || rcz >= 1) && (int64_t)g->C * std::max(g->frames, 1) == f->batch;
+}
+
+fr_status fr_fft_tw(fr_fft *f)
+{
+    if (f->tw) return FR_OK;
+    std::vector<float2> tw(f->N);
+    for (unsigned k = 0; k < f->N; ++k) {
+        const double a = 2.0 * M_PI * (double)k / (double)f->N * (double)f->sign;
+        tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    fr_status e = fr_alloc(f->ctx, (void **)&f->tw, sizeof(float2) * f->N);
+    if (e) return e;
+    FR_HIP(f->ctx, hipMemcpyAsync(f->tw, tw.data(), sizeof(float2) * f->N, hipMemcpyHostToDevice, f->ctx->stream));
+    FR_HIP(f->ctx, hipStreamSynchronize(f->ctx->stream));
+    return FR_OK;
+}
+
+extern "C" fr_status fr_fft_sos(fr_fft *f, fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz, int apply_shift,
+                                const float *deapod_dev, float *out_dev)
+{
+    if (!f || !g || !out_dev) return FR_ERR_INVALID;
+    fr_ctx *ctx = f->ctx;
+    if (g->dim == 2) rcz = g->frames > 0 ? (unsigned)g->frames : 1u;
+    FR_CHECK(ctx, g->dim == f->dim && g->N == f->N && (int64_t)g->C * std::max(g->frames, 1) == f->batch,
+             FR_ERR_INVALID, "grids (dim %d, N %u, C %d) do not match the FFT plan (dim %d, N %u, batch %d)", g->dim,
+             g->N, g->C, f->dim, f->N, f->batch);
+    FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
+             FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
+    FR_CHECK(ctx, !apply_shift || g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size");
+    if (!fr_fft_sos_fused(f, g, rcx, rcy, rcz)) {
+        // other sizes: the full batched FFT in place, then the fused epilogue
+        fr_status e = fr_fft_execute(f, g);
+        if (!e) e = fr_sos(ctx, g, rcx, rcy, rcz, apply_shift, deapod_dev, out_dev);
+        return e;
+    }
+    fr_status e = fr_fft_tw(f);
+    if (e) return e;
+    if (f->N == 256) return run_passes<256>(f, g, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev);
+    return run_passes<512>(f, g, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev);
+}

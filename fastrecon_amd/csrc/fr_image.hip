@@ -117,6 +117,8 @@ extern "C" fr_status fr_fft_destroy(fr_fft *f)
     if (f->info) rocfft_execution_info_destroy((rocfft_execution_info)f->info);
     if (f->plan) rocfft_plan_destroy((rocfft_plan)f->plan);
     hipFree(f->work);
+    hipFree(f->tw);
+    hipFree(f->crop_ws);
     delete f;
     return FR_OK;
 }

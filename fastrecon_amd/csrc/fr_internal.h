@@ -166,6 +166,10 @@ struct fr_fft {
     void *info = nullptr;              // rocfft_execution_info
     void *work = nullptr;
     size_t work_bytes = 0;
+    // cropped FFT + SOS path (fr_fftcrop.hip): twiddles and pass buffers
+    float2 *tw = nullptr;
+    void *crop_ws = nullptr;
+    size_t crop_ws_bytes = 0;
 };
 
 // ----------------------------------------------------------------- errors
@@ -202,4 +206,8 @@ static inline size_t fr_plan_cells(int dim, unsigned N, int frames)
 // records serve FR_ACC_MFMA when its LUT has at most FR_MFMA_MAX_L entries)
 fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode);
 fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out);
+// cropped FFT + SOS helpers (fr_fftcrop.hip)
+fr_status fr_fft_workspace(fr_fft *f, size_t bytes);
+fr_status fr_fft_tw(fr_fft *f);
+bool fr_fft_sos_fused(const fr_fft *f, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz);
 fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune);

@@ -27,7 +27,7 @@ EXPORTS = [
     "fr_grids_create", "fr_grids_create_frames", "fr_samples_create_frames", "fr_grids_destroy", "fr_grids_download", "fr_grids_upload", "fr_grids_device_ptr",
     "fr_grid_execute", "fr_degrid_execute",
     "fr_fft_create", "fr_fft_destroy", "fr_fft_execute",
-    "fr_fftshift", "fr_sos", "fr_deapod_map", "fr_lowfilter", "fr_normalize", "fr_crop", "fr_sos_map",
+    "fr_fftshift", "fr_sos", "fr_fft_sos", "fr_deapod_map", "fr_lowfilter", "fr_normalize", "fr_crop", "fr_sos_map",
     "fr_grids_copy",
     "fr_dcf_pipe_menon",
     "fr_graph_create", "fr_graph_launch", "fr_graph_destroy", "fr_autotune",
@@ -105,6 +105,7 @@ def load():
         "fr_fft_execute": [_vp, _vp],
         "fr_fftshift": [_vp, _vp],
         "fr_sos": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp],
+        "fr_fft_sos": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp],
         "fr_deapod_map": [_vp, C.c_int, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
         "fr_dcf_pipe_menon": [_vp, _vp, C.c_int, C.c_int],
         "fr_lowfilter": [_vp, _vp, C.c_int],
@@ -428,12 +429,27 @@ def fftshift(ctx, grids):
 def sos(ctx, grids, rc, shift=True, deapod=None, out=None):
     rcx, rcy, rcz = rc
     if grids.dim == 2:
-        rcz = 1
+        rcz = max(grids.frames, 1)       # batched frames: one image per frame
     n = rcx * rcy * rcz
     if out is None:
         out = DeviceBuffer(ctx, 4 * n)
     ctx.check(ctx.L.fr_sos(ctx.h, grids.h, rcx, rcy, rcz, 1 if shift else 0,
                            _vp(deapod.ptr) if deapod is not None else None, _vp(out.ptr)), "fr_sos")
+    return out
+
+
+def fft_sos(fft, grids, rc, shift=True, deapod=None, out=None):
+    """FFT + shift + crop + SOS in one call (fr_fft_sos); the grids' contents
+    afterwards are unspecified (untransformed on the cropped path)."""
+    ctx = fft.ctx
+    rcx, rcy, rcz = rc
+    if grids.dim == 2:
+        rcz = max(grids.frames, 1)
+    n = rcx * rcy * rcz
+    if out is None:
+        out = DeviceBuffer(ctx, 4 * n)
+    ctx.check(ctx.L.fr_fft_sos(fft.h, grids.h, rcx, rcy, rcz, 1 if shift else 0,
+                               _vp(deapod.ptr) if deapod is not None else None, _vp(out.ptr)), "fr_fft_sos")
     return out
 
 

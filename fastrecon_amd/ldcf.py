@@ -73,13 +73,13 @@ def pipe_menon_dcf(ctx, traj_n, N, iters=10):
 
 
 def recon_image(ctx, plan, samples, N, rc, fft=None, grids=None):
-    """grid -> FFT (+1) -> fftshift/crop/SOS, device resident; returns the host image."""
+    """grid -> FFT (+1) -> fftshift/crop/SOS, device resident (the cropped
+    transforms of fr_fft_sos); returns the host image."""
     dim = samples.dim
     g = grids or hip.Grids(ctx, dim, N, samples.C)
     f = fft or hip.FFT(ctx, dim, N, samples.C, +1)
     hip.grid_execute(plan, samples, g)
-    f(g)
-    img = hip.sos(ctx, g, rc, shift=True)
+    img = hip.fft_sos(f, g, rc, shift=True)
     out = img.to_host(np.float32)
     img.close()
     if grids is None:
@@ -227,8 +227,7 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
 
     def recon():
         hip.grid_execute(p, s, g)
-        f(g)
-        hip.sos(ctx, g, rc, True, None, img)
+        hip.fft_sos(f, g, rc, True, None, img)
 
     def iterative():
         hip.pipe_menon(p, s, iters, True)

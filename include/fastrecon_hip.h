@@ -189,6 +189,16 @@ fr_status fr_fft_create(fr_ctx *ctx, int dim, unsigned N, int batch, int sign, f
 fr_status fr_fft_destroy(fr_fft *f);
 fr_status fr_fft_execute(fr_fft *f, fr_grids *g);
 
+/* FFT + fftShift + crop + sum-of-squares in one call: out_dev receives what
+ * fr_fft_execute, then fr_sos(..., apply_shift, deapod_dev, out_dev) would
+ * give (FFT.cpp:82-84, ImageFilter.cpp:40-53,159-209, Main.cpp:175-181), to
+ * FP32 rounding.  For N = 256 or 512 and rcx a multiple of 16 it runs
+ * cropped transforms that keep only the rows and columns reaching the crop
+ * (the grids are left untransformed); otherwise it runs exactly those two
+ * calls (the grids are left transformed).  No counterpart in the reference. */
+fr_status fr_fft_sos(fr_fft *f, fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz, int apply_shift,
+                     const float *deapod_dev, float *out_dev);
+
 /* ---------------------------------------------------------- image stage */
 /* ImageFilter::fftShift (ImageFilter.cpp:40-53, 231-280), in place, even N. */
 fr_status fr_fftshift(fr_ctx *ctx, fr_grids *g);
@@ -231,9 +241,10 @@ fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones)
 /* ------------------------------------------------- graphs and autotuning
  * Net-new (the reference launches per coil and partition with a device
  * synchronisation after each, cuGridLut.cu:261-309).  fr_graph_create runs
- * one eager pass (gridding; the FFT if f; shift/crop/SOS into out_dev if
- * out_dev), then captures the same pass on the context's stream into a HIP
- * graph; fr_graph_launch replays it on the current sample data. */
+ * one eager pass (gridding; the FFT if f and no out_dev; fr_fft_sos into
+ * out_dev if out_dev, which leaves the grids' contents unspecified), then
+ * captures the same pass on the context's stream into a HIP graph;
+ * fr_graph_launch replays it on the current sample data. */
 typedef struct fr_graph fr_graph;
 fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *g, fr_fft *f, unsigned rcx, unsigned rcy,
                           unsigned rcz, int apply_shift, const float *deapod_dev, float *out_dev, fr_graph **out);

@@ -75,9 +75,12 @@ def test_batched_frames_equal_single_frames(ctx, orc, tune):
             assert rel_l2(got[:, f], ref) <= 1e-5
 
 
-def test_batched_recon_and_graph_replays(ctx, orc):
+@pytest.mark.parametrize("image", [32, 128])
+def test_batched_recon_and_graph_replays(ctx, orc, image):
+    """image 32 (N = 64): rocFFT + the SOS epilogue; image 128 (N = 256):
+    the cropped transforms with the coil sum in the last pass."""
     C = 3
-    N, lut, tns, dcfs, datas, image = frames_case(C=C, F=5)
+    N, lut, tns, dcfs, datas, image = frames_case(C=C, F=5, image=image)
     s, sizes = batched_samples(ctx, tns, dcfs, datas, C)
     F = len(tns)
     p = hip.Plan(ctx, s, N, 4.0, lut)

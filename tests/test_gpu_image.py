@@ -188,3 +188,63 @@ def test_pils_ops_bitwise(ctx, orc, dim, N, C):
     g2 = hip.Grids(ctx, dim, N, C)
     hip.grids_copy(ctx, g, g2)
     assert np.array_equal(g2.download_all(), d)
+
+
+def _rand_grids(ctx, dim, N, C, seed, frames=0):
+    rng = np.random.default_rng(seed)
+    g = hip.Grids(ctx, dim, N, C, frames=frames)
+    cells = N ** dim * max(frames, 1)
+    data = (rng.standard_normal((C, cells), np.float32)
+            + 1j * rng.standard_normal((C, cells), np.float32)).astype(np.complex64)
+    for c in range(C):
+        g.upload(c, data[c])
+    return g, data
+
+
+@pytest.mark.parametrize("dim,N,C,rc,shift,deapod,frames", [
+    (3, 256, 3, (128, 128, 128), True, False, 0),
+    (3, 256, 2, (96, 112, 100), True, True, 0),
+    (3, 256, 2, (128, 128, 128), False, False, 0),
+    (2, 512, 3, (256, 256, 1), True, False, 0),
+    (2, 512, 2, (192, 160, 1), False, True, 0),
+    (2, 256, 3, (128, 128, 1), True, False, 5),
+    (3, 128, 2, (64, 64, 64), True, False, 0),      # not a cropped-path size: the FFT + SOS fallback
+])
+def test_fft_sos_matches_fft_then_sos(ctx, dim, N, C, rc, shift, deapod, frames):
+    """fr_fft_sos (cropped transforms for N = 256, 512) equals fr_fft_execute
+    followed by fr_sos on the same grids to FP32 rounding."""
+    g, data = _rand_grids(ctx, dim, N, C, 31 + N + C, frames)
+    batch = C * max(frames, 1)
+    f = hip.FFT(ctx, dim, N, batch, +1)
+    rcz = rc[2] if dim == 3 else max(frames, 1)
+    dp = None
+    if deapod:
+        rng = np.random.default_rng(5)
+        dmap = rng.uniform(0.5, 2.0, rc[0] * rc[1] * (rc[2] if dim == 3 else 1)).astype(np.float32)
+        dp = hip.DeviceBuffer(ctx, 4 * dmap.size)
+        dp.from_host(dmap)
+    got = hip.fft_sos(f, g, rc, shift, dp).to_host(np.float32)
+    for c in range(C):
+        g.upload(c, data[c])
+    f(g)
+    ref = hip.sos(ctx, g, rc, shift, dp).to_host(np.float32)
+    assert got.size == rc[0] * rc[1] * rcz
+    e = rel_l2(got, ref)
+    print(f"[fft_sos {dim}D N={N} C={C} rc={rc} shift={shift} deapod={deapod} frames={frames}] "
+          f"rel-L2 vs FFT+SOS {e:.2e}", flush=True)
+    assert e <= 1e-6
+    for h in (f, g):
+        h.close()
+
+
+def test_fft_sos_against_oracle(ctx, orc):
+    """The cropped path against the FP64 oracle FFT, fftshift and SOS."""
+    N, C, rc = 256, 1, (128, 128, 128)
+    g, data = _rand_grids(ctx, 3, N, C, 77)
+    f = hip.FFT(ctx, 3, N, C, +1)
+    got = hip.fft_sos(f, g, rc, True).to_host(np.float32)
+    img = orc.fftshift(3, N, orc.fft_c2c(3, N, data, +1))
+    ref = orc.sos(3, N, img, rc)
+    e = rel_l2(got, ref)
+    print(f"[fft_sos vs FP64 oracle] rel-L2 {e:.2e}", flush=True)
+    assert e <= TOL
