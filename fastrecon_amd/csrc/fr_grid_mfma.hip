@@ -281,10 +281,16 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
         // LDS as [component][cell], then consecutive lanes store consecutive
         // cells of one coil, so each store covers whole rows of the unit (ux
         // contiguous cells) instead of one 8-byte value in each of 32 lines.
+        // When the unit's cell count divides 64 (the usual full units of 16,
+        // 32 or 64 cells) every lane keeps one cell for the whole write-out
+        // and steps through the coils; otherwise cell offsets go through LDS.
+        const bool lane_cell = (64 % ncell) == 0;
         __syncthreads();
-        for (int q = lane; q < ncell; q += 64) {
-            const int lx = q % ux, ly = (q / ux) % uy, lz = q / (ux * uy);
-            goff[q] = (uint32_t)(((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx));
+        if (!lane_cell) {
+            for (int q = lane; q < ncell; q += 64) {
+                const int lx = q % ux, ly = (q / ux) % uy, lz = q / (ux * uy);
+                goff[q] = (uint32_t)(((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx));
+            }
         }
 #pragma unroll
         for (int b = 0; b < NBLK; ++b)
@@ -293,17 +299,31 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
 #pragma unroll
                 for (int r = 0; r < 4; ++r) osm[(NCB * j + cb) * CS + 16 * b + 4 * k + r] = acc[b][cb][r];
         __syncthreads();
-        const int pairs = nc * ncell;
-        for (int t = lane; t < pairs; t += 64) {
-            const int cl = t / ncell;
-            const int q = t - cl * ncell;
+        // each grid value is written once and not read again by this pass: a
+        // streaming store keeps the sample rows in L2
+        auto put = [&](int cl, int q, size_t go) {
             const float2 v = make_float2(osm[(2 * cl) * CS + q], osm[(2 * cl + 1) * CS + q]);
             const int c = c0 + cl;
             if (it.slot < 0) {
-                out[(size_t)c * cells + goff[q]] = v;
+                __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v),
+                                            reinterpret_cast<unsigned long long *>(out + (size_t)c * cells + go));
             } else {
                 const size_t slot = ((size_t)(c >> 5) * nslots + it.slot) * 32 + (c & 31);
                 partial[slot * cells_max + q] = v;
+            }
+        };
+        if (lane_cell) {
+            const int sh = __builtin_ctz((unsigned)ncell);     // ncell is a power of two here
+            const int q = lane & (ncell - 1);
+            const int lx = q % ux, ly = (q / ux) % uy, lz = q / (ux * uy);
+            const size_t go = ((size_t)(U.oz + lz) * N + (U.oy + ly)) * N + (U.ox + lx);
+            for (int cl = lane >> sh; cl < nc; cl += 64 >> sh) put(cl, q, go);
+        } else {
+            const int pairs = nc * ncell;
+            for (int t = lane; t < pairs; t += 64) {
+                const int cl = t / ncell;
+                const int q = t - cl * ncell;
+                put(cl, q, goff[q]);
             }
         }
         __syncthreads();

@@ -121,11 +121,10 @@ __device__ inline uint32_t box_count(const uint32_t *__restrict__ S, int dim, in
            sat_at(S, N, hi[0], lo[1], lo[2]) - sat_at(S, N, lo[0], lo[1], lo[2]);
 }
 
-__global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, const uint32_t *__restrict__ S,
-                           float *__restrict__ feat)
+// the six features of sample s (f[6], f[7] = 0)
+__device__ inline void sample_features(const float *__restrict__ traj, int64_t M, int dim, int N,
+                                       const uint32_t *__restrict__ S, int64_t s, float (&f)[FSTRIDE])
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
     const float cf = 0.5f * (float)(N - 1), hN = 0.5f * (float)N;
     float u[3] = {0.f, 0.f, 0.f};
     int c[3] = {0, 0, 0};
@@ -138,7 +137,6 @@ __global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, i
                               : __fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1]));
     const float r = sqrtf(r2);
     const float inv_pi = 0.318309886183790671f;
-    float f[FSTRIDE];
     f[0] = log1pf(r * hN) * 0.2f;             // radius in cells, log-compressed
     f[1] = atan2f(u[1], u[0]) * inv_pi;
     f[2] = (dim == 3 && r > 0.f) ? acosf(fminf(fmaxf(u[2] / r, -1.f), 1.f)) * inv_pi : 0.5f;
@@ -146,6 +144,15 @@ __global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, i
     for (int q = 0; q < 3; ++q) f[3 + q] = log1pf((float)(box_count(S, dim, N, c, rr[q]) - 1u)) * 0.125f;
     f[6] = 0.f;
     f[7] = 0.f;
+}
+
+__global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, const uint32_t *__restrict__ S,
+                           float *__restrict__ feat)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    float f[FSTRIDE];
+    sample_features(traj, M, dim, N, S, s, f);
     float4 *o = reinterpret_cast<float4 *>(feat + s * FSTRIDE);
     o[0] = make_float4(f[0], f[1], f[2], f[3]);
     o[1] = make_float4(f[4], f[5], f[6], f[7]);
@@ -184,7 +191,45 @@ struct fwd_state {
     float y;
 };
 
-__device__ inline void mlp_forward16(const mlp_dev &W, const u32x4 (&a)[4][2], const float *x, int lane, fwd_state &F)
+// Where the forward reads the layer-1 weights, the biases and w3: straight
+// from the device parameters, or from a workgroup's LDS copy (same values).
+struct wsrc_global {
+    const mlp_dev &W;
+    __device__ float w1(int i, int k) const { return bf2f(W.w1[i * H + k]); }
+    __device__ float b1(int k) const { return W.p[O_B1 + k]; }
+    __device__ float b2(int o) const { return W.p[O_B2 + o]; }
+    __device__ float w3(int o) const { return bf2f(W.w3[o]); }
+    __device__ float b3() const { return W.p[O_B3]; }
+};
+
+struct lds_weights {
+    float w1[NF][H];
+    float b1[H], b2[H], w3[H];
+    float b3;
+};
+
+struct wsrc_lds {
+    const lds_weights &L;
+    __device__ float w1(int i, int k) const { return L.w1[i][k]; }
+    __device__ float b1(int k) const { return L.b1[k]; }
+    __device__ float b2(int o) const { return L.b2[o]; }
+    __device__ float w3(int o) const { return L.w3[o]; }
+    __device__ float b3() const { return L.b3; }
+};
+
+__device__ inline void stage_weights(const mlp_dev &W, lds_weights &L)
+{
+    for (int q = threadIdx.x; q < NF * H; q += blockDim.x) L.w1[q / H][q % H] = bf2f(W.w1[q]);
+    for (int q = threadIdx.x; q < H; q += blockDim.x) {
+        L.b1[q] = W.p[O_B1 + q];
+        L.b2[q] = W.p[O_B2 + q];
+        L.w3[q] = bf2f(W.w3[q]);
+    }
+    if (threadIdx.x == 0) L.b3 = W.p[O_B3];
+}
+
+template <class WS>
+__device__ inline void mlp_forward16(const WS &W, const u32x4 (&a)[4][2], const float *x, int lane, fwd_state &F)
 {
     const int g = lane >> 4;
     u32x4 b[2];
@@ -194,9 +239,9 @@ __device__ inline void mlp_forward16(const mlp_dev &W, const u32x4 (&a)[4][2], c
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = 32 * ks + 8 * g + j;
-            float z = W.p[O_B1 + k];
+            float z = W.b1(k);
 #pragma unroll
-            for (int i = 0; i < NF; ++i) z = __fadd_rn(z, __fmul_rn(x[i], bf2f(W.w1[i * H + k])));
+            for (int i = 0; i < NF; ++i) z = __fadd_rn(z, __fmul_rn(x[i], W.w1(i, k)));
             F.z1[ks][j] = z;
             hb[j] = f2bf(silu(z));
             F.h1[ks][j] = bf2f(hb[j]);
@@ -214,16 +259,16 @@ __device__ inline void mlp_forward16(const mlp_dev &W, const u32x4 (&a)[4][2], c
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int o = 16 * mt + 4 * g + i;
-            const float z = __fadd_rn(acc[i], W.p[O_B2 + o]);
+            const float z = __fadd_rn(acc[i], W.b2(o));
             F.z2[mt][i] = z;
             F.h2[mt][i] = silu(z);
-            part = __fadd_rn(part, __fmul_rn(F.h2[mt][i], bf2f(W.w3[o])));
+            part = __fadd_rn(part, __fmul_rn(F.h2[mt][i], W.w3(o)));
         }
     }
     // the four lane groups hold the partial sums of the same sample
     const float p1 = __fadd_rn(part, __shfl_xor(part, 16));
     const float p2 = __fadd_rn(p1, __shfl_xor(p1, 32));
-    F.y = __fadd_rn(p2, W.p[O_B3]);
+    F.y = __fadd_rn(p2, W.b3());
 }
 
 __device__ inline void load_x(const float *__restrict__ feat, int64_t row, float *x)
@@ -233,23 +278,76 @@ __device__ inline void load_x(const float *__restrict__ feat, int64_t row, float
     x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y;
 }
 
+__device__ inline float mlp_out(float y, int mode, float mu, float sigma)
+{
+    return mode == 0 ? y : __expf(__fadd_rn(mu, __fmul_rn(sigma, y)));
+}
+
 // y for every sample; mode 0 writes y, mode 1 writes exp(mu + sigma y)
 __global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__restrict__ feat, int64_t M, int mode,
                                                      float mu, float sigma, float *__restrict__ out)
 {
+    __shared__ lds_weights LW;
+    stage_weights(W, LW);
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     u32x4 a[4][2];
     load_a(W, lane, a);
+    __syncthreads();
+    const wsrc_lds ws{LW};
     const int64_t groups = (M + 15) / 16;
     for (int64_t gi = wave; gi < groups; gi += nwaves) {
         const int64_t s = gi * 16 + (lane & 15);
         float x[NF];
         load_x(feat, s < M ? s : M - 1, x);
         fwd_state F;
-        mlp_forward16(W, a, x, lane, F);
-        if (s < M && lane < 16) out[s] = mode == 0 ? F.y : __expf(__fadd_rn(mu, __fmul_rn(sigma, F.y)));
+        mlp_forward16(ws, a, x, lane, F);
+        if (s < M && lane < 16) out[s] = mlp_out(F.y, mode, mu, sigma);
+    }
+}
+
+// Features and forward in one pass (no feature rows in memory): each wave
+// takes 64 samples, lane l computes the features of sample l into LDS, then
+// the wave runs the network over them 16 at a time.  Same arithmetic as
+// k_features followed by k_mlp_forward.
+__global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__restrict__ traj, int64_t M, int dim, int N,
+                                                  const uint32_t *__restrict__ S, int mode, float mu, float sigma,
+                                                  float *__restrict__ out)
+{
+    __shared__ lds_weights LW;
+    __shared__ float xs[4][64][NF + 1];
+    stage_weights(W, LW);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    u32x4 a[4][2];
+    load_a(W, lane, a);
+    __syncthreads();
+    const wsrc_lds ws{LW};
+    const int64_t groups = (M + 63) / 64;
+    for (int64_t gi = wave; gi < groups; gi += nwaves) {
+        const int64_t s0 = gi * 64;
+        {
+            float f[FSTRIDE];
+            sample_features(traj, M, dim, N, S, min(s0 + lane, M - 1), f);
+#pragma unroll
+            for (int i = 0; i < NF; ++i) xs[wv][lane][i] = f[i];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * q + (lane & 15);
+            float x[NF];
+#pragma unroll
+            for (int i = 0; i < NF; ++i) x[i] = xs[wv][r][i];
+            fwd_state F;
+            mlp_forward16(ws, a, x, lane, F);
+            const int64_t s = s0 + r;
+            if (s < M && lane < 16) out[s] = mlp_out(F.y, mode, mu, sigma);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -300,7 +398,7 @@ __global__ void __launch_bounds__(256) k_mlp_backward(mlp_dev W, const float *__
         float x[NF];
         load_x(feat, row, x);
         fwd_state F;
-        mlp_forward16(W, a, x, lane, F);
+        mlp_forward16(wsrc_global{W}, a, x, lane, F);
         const float r = valid ? __fsub_rn(F.y, targ[row]) : 0.f;
         const float dy = __fmul_rn(inv2B, r);
         // dz2 for the lane's outputs; stage the per-sample rows in LDS
@@ -416,6 +514,8 @@ struct fr_mlp {
     int64_t partial_slots = 0;
     int64_t step = 0;
     float mu = 0.f, sigma = 1.f;
+    uint32_t *hist = nullptr;     // cell histogram / summed-area table of fr_ldcf_dcf, kept between calls
+    size_t hist_cells = 0;
 };
 
 namespace {
@@ -423,6 +523,15 @@ mlp_dev view(const fr_mlp *m) { return mlp_dev{m->p, m->w1, m->w2, m->w3}; }
 }
 
 extern "C" int fr_mlp_param_count(void) { return NPARAM; }
+
+// zeroed histogram -> per-cell counts -> summed-area table (in place)
+static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32_t *hist)
+{
+    hipLaunchKernelGGL(k_cell_hist, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist);
+    const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
+    for (int ax = 0; ax < s->dim; ++ax)
+        hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, hist, (int)N, s->dim, ax);
+}
 
 extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev)
 {
@@ -437,10 +546,7 @@ extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned
     hipStream_t q = ctx->stream;
     hipError_t e = hipMemsetAsync(hist, 0, cells * sizeof(uint32_t), q);
     if (e == hipSuccess && s->M > 0) {
-        hipLaunchKernelGGL(k_cell_hist, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist);
-        const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
-        for (int ax = 0; ax < s->dim; ++ax)
-            hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, hist, (int)N, s->dim, ax);
+        launch_counts(q, s, N, hist);
         hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist,
                            feat_dev);
         e = hipGetLastError();
@@ -479,6 +585,7 @@ extern "C" void fr_mlp_destroy(fr_mlp *m)
     if (m->ctx) hipStreamSynchronize(m->ctx->stream);
     hipFree(m->p); hipFree(m->m); hipFree(m->v); hipFree(m->grad);
     hipFree(m->w1); hipFree(m->w2); hipFree(m->w3); hipFree(m->partial);
+    hipFree(m->hist);
     delete m;
 }
 
@@ -525,6 +632,36 @@ extern "C" fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M,
     hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, M)), dim3(256), 0, ctx->stream, view(m), feat_dev, M, 0,
                        0.f, 1.f, y_dev);
     FR_HIP(ctx, hipGetLastError());
+    return FR_OK;
+}
+
+extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
+{
+    if (!m || !s) return FR_ERR_INVALID;
+    fr_ctx *ctx = m->ctx;
+    FR_CHECK(ctx, s->has_traj, FR_ERR_STATE, "trajectory not set");
+    FR_CHECK(ctx, s->frames == 0, FR_ERR_UNSUPPORTED, "learned DCF of batched frames: compute it per frame");
+    FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
+    const size_t cells = fr_cells(s->dim, N);
+    if (cells > m->hist_cells) {
+        FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        hipFree(m->hist);
+        m->hist = nullptr;
+        m->hist_cells = 0;
+        fr_status st = fr_alloc(ctx, (void **)&m->hist, cells * sizeof(uint32_t));
+        if (st) return st;
+        m->hist_cells = cells;
+    }
+    if (s->M > 0) {
+        FR_HIP(ctx, hipMemsetAsync(m->hist, 0, cells * sizeof(uint32_t), ctx->stream));
+        launch_counts(ctx->stream, s, N, m->hist);
+        const int64_t groups = (s->M + 63) / 64;
+        const int64_t waves = std::min<int64_t>(groups, (int64_t)std::max(ctx->cu_count, 1) * 32);
+        hipLaunchKernelGGL(k_ldcf_dcf, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0,
+                           ctx->stream, view(m), s->traj, s->M, s->dim, (int)N, m->hist, 1, m->mu, m->sigma, s->dcf);
+    }
+    FR_HIP(ctx, hipGetLastError());
+    s->has_dcf = true;
     return FR_OK;
 }
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "fr_dcf_pipe_menon",
     "fr_graph_create", "fr_graph_launch", "fr_graph_destroy", "fr_autotune",
     "fr_ldcf_features", "fr_mlp_param_count", "fr_mlp_create", "fr_mlp_destroy", "fr_mlp_set_params",
-    "fr_mlp_get_params", "fr_mlp_forward", "fr_mlp_dcf", "fr_mlp_grad", "fr_mlp_train_step",
+    "fr_mlp_get_params", "fr_mlp_forward", "fr_mlp_dcf", "fr_ldcf_dcf", "fr_mlp_grad", "fr_mlp_train_step",
     "fr_malloc", "fr_free", "fr_memcpy_d2h", "fr_memcpy_h2d",
 ]
 
@@ -120,6 +120,7 @@ def load():
         "fr_mlp_get_params": [_vp, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "fr_mlp_forward": [_vp, _vp, C.c_int64, _vp],
         "fr_mlp_dcf": [_vp, _vp, _vp],
+        "fr_ldcf_dcf": [_vp, _vp, C.c_uint],
         "fr_mlp_grad": [_vp, _vp, _vp, _vp, C.c_int64, _vp, C.POINTER(C.c_float)],
         "fr_mlp_train_step": [_vp, _vp, _vp, _vp, C.c_int64, C.c_float, C.POINTER(C.c_float)],
         "fr_malloc": [_vp, C.c_size_t, _pp],
@@ -539,6 +540,10 @@ class MLP:
 
     def dcf(self, feat, samples):
         self.ctx.check(self.ctx.L.fr_mlp_dcf(self.h, _vp(feat.ptr), samples.h), "fr_mlp_dcf")
+
+    def dcf_from_traj(self, samples, N):
+        """Features + network in one pass (fr_ldcf_dcf): the samples' DCF from their trajectory."""
+        self.ctx.check(self.ctx.L.fr_ldcf_dcf(self.h, samples.h, int(N)), "fr_ldcf_dcf")
 
     def grad(self, feat, targ, B, idx=None):
         g = np.zeros(self.nparam, np.float32)

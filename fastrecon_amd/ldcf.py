@@ -187,10 +187,9 @@ def load(ctx, path=CHECKPOINT):
 
 
 def learned_dcf(ctx, mlp, samples, N):
-    """Features of the samples' (normalised) trajectory -> MLP -> samples' DCF."""
-    feat = hip.ldcf_features(ctx, samples, N)
-    mlp.dcf(feat, samples)
-    feat.close()
+    """Features of the samples' (normalised) trajectory -> MLP -> samples' DCF
+    (one fused pass, fr_ldcf_dcf)."""
+    mlp.dcf_from_traj(samples, N)
 
 
 def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None, coils=None):
@@ -234,10 +233,8 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
         recon()
 
     def learned():
-        feat = hip.ldcf_features(ctx, s, N)
-        mlp.dcf(feat, s)
+        mlp.dcf_from_traj(s, N)
         recon()
-        feat.close()
 
     t_it = timed(iterative)
     im_it = img.to_host(np.float32)
@@ -249,11 +246,9 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
 
     def learned_refined():
         # the learned DCF as the start of one Pipe-Menon iteration
-        feat = hip.ldcf_features(ctx, s, N)
-        mlp.dcf(feat, s)
+        mlp.dcf_from_traj(s, N)
         hip.pipe_menon(p, s, 1, False)
         recon()
-        feat.close()
 
     t_mr = timed(learned_refined)
     im_mr = img.to_host(np.float32)

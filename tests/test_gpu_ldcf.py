@@ -49,6 +49,32 @@ def test_features_match_oracle(ctx, dim, N, M):
     assert np.all(f[:, 6:] == 0)
 
 
+@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41)])
+def test_fused_dcf_equals_features_then_network(ctx, dim, N, M):
+    """fr_ldcf_dcf (features in LDS, no feature rows) is bitwise the DCF of
+    fr_ldcf_features + fr_mlp_dcf; M below, at and off multiples of 64, and a
+    second call on a smaller grid reuses the kept workspace."""
+    rng = np.random.default_rng(dim * 100 + M)
+    t = np.concatenate([rng.normal(N / 2, 2.5, (M // 2, dim)), rng.uniform(-0.7, N - 0.3, (M - M // 2, dim))])
+    t = t.astype(np.float32)
+    net = hip.MLP(ctx)
+    net.set_params(random_net(dim + M), 0.3, 0.7)
+    for n in (N, N // 2 + 3):
+        s = samples_for(ctx, t)
+        feat = hip.ldcf_features(ctx, s, n)
+        net.dcf(feat, s)
+        want = s.get_dcf()
+        s.set_dcf(np.zeros(M, np.float32))
+        net.dcf_from_traj(s, n)
+        got = s.get_dcf()
+        print(f"[fused dcf dim={dim} N={n} M={M}] equal {np.array_equal(got.view(np.int32), want.view(np.int32))}",
+              flush=True)
+        np.testing.assert_array_equal(got.view(np.int32), want.view(np.int32))
+        feat.close()
+        s.close()
+    net.close()
+
+
 def random_net(seed):
     rng = np.random.default_rng(seed)
     return (lo.init_params(seed) + 0.05 * rng.standard_normal(lo.NPARAM)).astype(np.float32)
