@@ -62,20 +62,28 @@ __device__ __forceinline__ float4 ld_pair(const __half2 *__restrict__ row, int c
 constexpr int RB = 16;                       // records per batch
 constexpr int64_t XCD_RUN = FR_MFMA_XCD_RUN; // consecutive items one XCD takes in turn
 constexpr int TAP_ROW = 4 * FR_ACC_STRIDE;   // plan taps: (cell * TAP_ROW, LUT weight bits)
-// the pipeline reads headers up to 2 RB + 15 records past a live record; the
-// plan pads H by 64 records
-static_assert(2 * RB + 15 < 64, "record header padding");
+// the pipeline reads headers up to 3 RB - 1 records past a live record; the
+// plan pads the headers and tap starts by 64 records
+static_assert(3 * RB - 1 < 64, "record header padding");
 
-// One batch of RB records in flight.  Lane l < RB of h/n/excl/w describes
-// record l of the batch; tap pass p gives lane l tap u = l % TPR of record
-// p * RPP + l / TPR (a record carries at most TPR = 16 NBLK taps); rows:
-// group q gives lane (j, k) the component slice of record 4 q + k.
+// Every lane reads the headers of the records it works on (16 lanes share
+// each one), so a batch needs no cross-lane exchange: tap pass p gives lane l
+// tap u = l % TPR of record p * RPP + l / TPR (a record carries at most TPR =
+// 16 NBLK taps), and row group q gives lane (j, k) the DCF and component
+// slice of record 4 q + k (for NBLK = 1 the same record as pass q).
+template <int NBLK>
+struct hdr_st {
+    uint2 hp[4 * NBLK];         // pass p: (sample, padded tap count) of the lane's record
+    uint32_t t0[4 * NBLK];      // ... and the index of its first tap
+    uint2 hg[NBLK == 1 ? 1 : 4];   // group q (NBLK > 1): (sample, count) of record 4 q + k
+};
+
+// One batch of RB records in flight: taps (padding taps where the record has
+// no more), DCFs (0 past the item's end) and sample rows.
 template <int NBLK, bool CT>
 struct batch_st {
-    uint32_t s;                 // lane l < RB: sample of record l
-    uint32_t n, excl;           // its (padded) tap count and first tap within the batch
-    float w;                    // its DCF (0 past the item's end)
     typename std::conditional<CT, uint32_t, uint2>::type tp[4 * NBLK];   // compact or full taps
+    float w[4];
     float2 a[4], b[4];
 };
 
@@ -85,37 +93,55 @@ __device__ __forceinline__ int tap_cell(uint2 t) { return (int)(t.x / TAP_ROW); 
 __device__ __forceinline__ float tap_weight(uint32_t t, const float *slut) { return slut[t >> 6]; }
 __device__ __forceinline__ float tap_weight(uint2 t, const float *) { return __uint_as_float(t.y); }
 
-template <int NBLK, int NCB, bool PAIR, bool CT, typename TS, typename TT>
-__device__ __forceinline__ uint32_t issue_batch(batch_st<NBLK, CT> &S, uint2 h, uint32_t rb, uint32_t r1,
-                                                uint32_t tbase, const TT *__restrict__ T, const TS *__restrict__ data,
-                                                const float *__restrict__ dcf, uint32_t C, int ca, int cbb, int lane)
+// element at a 32-bit byte offset from a uniform base: the load takes the base
+// from SGPRs and only the offset from a VGPR (no 64-bit address arithmetic);
+// the plan arrays and the DCF stay below 4 GiB (checked by the launcher)
+template <typename E>
+__device__ __forceinline__ E ld_off(const E *__restrict__ base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const E *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+template <int NBLK>
+__device__ __forceinline__ void load_hdrs(hdr_st<NBLK> &G, uint32_t rb, const uint2 *__restrict__ H,
+                                          const uint32_t *__restrict__ tstart, int lane)
 {
     constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
-    const bool live = lane < RB && rb + (uint32_t)lane < r1;
-    const uint32_t n = live ? h.y : 0u;
-    uint32_t pre = n;                           // inclusive prefix over lanes 0 .. RB-1
-#pragma unroll
-    for (int d = 1; d < RB; d <<= 1) {
-        const uint32_t t = (uint32_t)__shfl_up((int)pre, d, RB);
-        if ((lane & (RB - 1)) >= d) pre += t;
-    }
-    S.s = h.x;
-    S.n = n;
-    S.excl = pre - n;
-    const float wd = dcf[h.x];
-    S.w = live ? wd : 0.f;
 #pragma unroll
     for (int p = 0; p < 4 * NBLK; ++p) {
-        const int rho = p * RPP + lane / TPR;
-        const uint32_t u = (uint32_t)(lane % TPR);
-        const uint32_t nr = (uint32_t)__shfl((int)n, rho);
-        const uint32_t off = (uint32_t)__shfl((int)(pre - n), rho);
-        S.tp[p] = T[u < nr ? tbase + off + u : tbase];
+        const uint32_t rec = rb + p * RPP + lane / TPR;
+        G.hp[p] = ld_off(H, rec * (uint32_t)sizeof(uint2));
+        G.t0[p] = ld_off(tstart, rec * (uint32_t)sizeof(uint32_t));
+    }
+    if constexpr (NBLK > 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) G.hg[q] = ld_off(H, (rb + 4 * q + (lane >> 4)) * (uint32_t)sizeof(uint2));
+    }
+}
+
+// Nothing here uses a loaded value (only addresses), so the loads of the
+// next batch stay in flight while the current one is applied: a lane with no
+// tap left reads a padding tap from the tail of the tap array (index padidx,
+// the plan's tap count), and the DCF of records past the item's end is masked
+// when the batch is applied.
+template <int NBLK, int NCB, bool PAIR, bool CT, typename TS, typename TT>
+__device__ __forceinline__ void issue_batch(batch_st<NBLK, CT> &S, const hdr_st<NBLK> &G, uint32_t rb, uint32_t r1,
+                                            const TT *__restrict__ T, uint32_t padidx, const TS *__restrict__ data,
+                                            const float *__restrict__ dcf, uint32_t C, int ca, int cbb, int lane)
+{
+    constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
+    const uint32_t u = (uint32_t)(lane % TPR);
+#pragma unroll
+    for (int p = 0; p < 4 * NBLK; ++p) {
+        const uint32_t rec = rb + p * RPP + lane / TPR;
+        const uint32_t n = rec < r1 ? G.hp[p].y : 0u;
+        S.tp[p] = ld_off(T, (u < n ? G.t0[p] + u : padidx) * (uint32_t)sizeof(TT));
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint32_t sq = (uint32_t)__shfl((int)h.x, 4 * q + (lane >> 4));
-        const TS *row = data + (uint64_t)sq * C;
+        const uint2 h = NBLK == 1 ? G.hp[q] : G.hg[NBLK == 1 ? 0 : q];
+        S.w[q] = ld_off(dcf, h.x * (uint32_t)sizeof(float));
+        const TS *row = data + (uint64_t)h.x * C;
         if constexpr (NCB == 4 && PAIR) {
             const float4 v = ld_pair(row, ca);
             S.a[q] = make_float2(v.x, v.y);
@@ -125,36 +151,38 @@ __device__ __forceinline__ uint32_t issue_batch(batch_st<NBLK, CT> &S, uint2 h, 
             if constexpr (NCB == 4) S.b[q] = ld_coil(row, cbb);
         }
     }
-    return (uint32_t)__builtin_amdgcn_readlane((int)pre, RB - 1);
 }
 
-template <int NBLK, int NCB, bool CT>
+template <int NBLK, int NCB, bool CT, bool FULL>
 __device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st<NBLK, CT> &S, float *tile,
-                                            const float *slut, bool v0ok, bool v1ok, int lane)
+                                            const float *slut, bool v0ok, bool v1ok, uint32_t rb, uint32_t r1,
+                                            int lane)
 {
     constexpr int TPR = 16 * NBLK, RPP = 64 / TPR;
     constexpr int ROW = 16 * NBLK + 1;          // cells (and the padding taps' row) per record column
     const int j = lane & 15, k = lane >> 4;
     // A tile [record][cell]: zero, then every tap writes its LUT weight
-    // (distinct cells within a record; padding taps write 0 to row cells_max)
+    // (the tile is allocated in whole multiples of 64 floats, so the zeroing
+    // needs no bounds test and no branch)
 #pragma unroll
-    for (int z0 = 0; z0 < RB * ROW; z0 += 64)
-        if (z0 + lane < RB * ROW) tile[z0 + lane] = 0.f;
+    for (int z0 = 0; z0 < RB * ROW; z0 += 64) tile[z0 + lane] = 0.f;
 #pragma unroll
     for (int p = 0; p < 4 * NBLK; ++p) {
+        // taps of one record have distinct cells; padding taps all write 0
+        // to the spare column cells_max, which no output reads
         const int rho = p * RPP + lane / TPR;
-        const uint32_t nr = (uint32_t)__shfl((int)S.n, rho);
-        if ((uint32_t)(lane % TPR) < nr) tile[rho * ROW + tap_cell(S.tp[p])] = tap_weight(S.tp[p], slut);
+        tile[rho * ROW + tap_cell(S.tp[p])] = tap_weight(S.tp[p], slut);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int rec = 4 * q + k;
-        // B = dcf * sample (GridLut.cpp:109); padding coil columns get 0
-        const float w = __shfl(S.w, rec);
-        const float w0 = v0ok ? w : 0.f;
+        // B = dcf * sample (GridLut.cpp:109); padding coil columns and
+        // records past the item's end get 0
+        const float w = rb + (uint32_t)rec < r1 ? S.w[q] : 0.f;
+        const float w0 = FULL || v0ok ? w : 0.f;
         float bv[NCB];
         if constexpr (NCB == 4) {
-            const float w1 = v1ok ? w : 0.f;
+            const float w1 = FULL || v1ok ? w : 0.f;
             bv[0] = __fmul_rn(w0, S.a[q].x); bv[1] = __fmul_rn(w0, S.a[q].y);
             bv[2] = __fmul_rn(w1, S.b[q].x); bv[3] = __fmul_rn(w1, S.b[q].y);
         } else if constexpr (NCB == 2) {
@@ -177,14 +205,15 @@ __device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st
 // aligned 16-byte (8-byte for FP16) load.
 // CT: the plan's taps are compact (uint16_t: cell | LUT index << 6) and the
 // weights come from the LUT staged in LDS (entry L is 0, for padding taps).
-template <int NBLK, int NCB, bool PAIR, bool CT, typename TS>
+template <int NBLK, int NCB, bool PAIR, bool CT, bool FULL, typename TS>
 __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ items, int64_t nitems,
                                                   const fr_unit *__restrict__ units, const uint2 *__restrict__ H,
+                                                  const uint32_t *__restrict__ tstart,
                                                   const void *__restrict__ Tv, const TS *__restrict__ data,
                                                   const float *__restrict__ dcf, int C, int N, size_t cells,
                                                   float2 *__restrict__ out, float2 *__restrict__ partial,
                                                   int64_t nslots, int cells_max, const float *__restrict__ lut,
-                                                  int L, int64_t nfront)
+                                                  int L, int64_t nfront, uint32_t padidx)
 {
     using TT = typename std::conditional<CT, uint16_t, uint2>::type;
     const TT *__restrict__ T = reinterpret_cast<const TT *>(Tv);
@@ -195,7 +224,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     }
     // LDS: the A tile during the record loop, then the write-out transpose
     constexpr int CS = 16 * NBLK + 1;           // cells per component row of the transpose
-    constexpr int TILE = RB * (16 * NBLK + 1), OSM = 16 * NCB * CS;
+    constexpr int TILE = (RB * (16 * NBLK + 1) + 63) / 64 * 64, OSM = 16 * NCB * CS;
     __shared__ float smem[(TILE > OSM ? TILE : OSM)];
     __shared__ uint32_t goff[16 * NBLK];        // grid offset of each unit cell
     float *tile = smem, *osm = smem;
@@ -244,35 +273,31 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
             // Software pipeline over batches of RB records: while batch b is
             // applied, batch b+1's taps, DCFs and sample rows and batch b+2's
             // headers are in flight.  Records past the item's end read valid
-            // (padded) headers and are masked (no taps, zero DCF).
+            // (padded) headers and are masked (padding taps, zero DCF).
             const uint32_t r1 = it.r1;
-            uint32_t r = it.r0, tb = it.t0;
-            const uint32_t hl = (uint32_t)(lane & (RB - 1));
+            uint32_t r = it.r0;
+            hdr_st<NBLK> G0, G1;
             batch_st<NBLK, CT> S0, S1;
-            uint32_t T0 = issue_batch<NBLK, NCB, PAIR, CT>(S0, H[r + hl], r, r1, tb, T, data, dcf, (uint32_t)C, ca, cbb,
-                                                       lane);
-            uint2 h1 = H[r + RB + hl], h3;
-            uint32_t T1;
+            load_hdrs<NBLK>(G0, r, H, tstart, lane);
+            load_hdrs<NBLK>(G1, r + RB, H, tstart, lane);
+            issue_batch<NBLK, NCB, PAIR, CT>(S0, G0, r, r1, T, padidx, data, dcf, (uint32_t)C, ca, cbb, lane);
             while (true) {
-                const uint2 h2 = H[r + 2 * RB + hl];
-                T1 = issue_batch<NBLK, NCB, PAIR, CT>(S1, h1, r + RB, r1, tb + T0, T, data, dcf, (uint32_t)C, ca, cbb,
-                                                  lane);
+                load_hdrs<NBLK>(G0, r + 2 * RB, H, tstart, lane);
+                issue_batch<NBLK, NCB, PAIR, CT>(S1, G1, r + RB, r1, T, padidx, data, dcf, (uint32_t)C, ca, cbb,
+                                                 lane);
                 // keep the prefetch ahead of the compute (the scheduler would
                 // otherwise sink each load to its use and wait on it there)
                 __builtin_amdgcn_sched_barrier(0);
-                apply_batch<NBLK, NCB, CT>(acc, S0, tile, slut, v0ok, v1ok, lane);
+                apply_batch<NBLK, NCB, CT, FULL>(acc, S0, tile, slut, v0ok, v1ok, r, r1, lane);
                 r += RB;
-                tb += T0;
                 if (r >= r1) break;
-                h3 = H[r + 2 * RB + hl];
-                T0 = issue_batch<NBLK, NCB, PAIR, CT>(S0, h2, r + RB, r1, tb + T1, T, data, dcf, (uint32_t)C, ca, cbb,
-                                                  lane);
+                load_hdrs<NBLK>(G1, r + 2 * RB, H, tstart, lane);
+                issue_batch<NBLK, NCB, PAIR, CT>(S0, G0, r + RB, r1, T, padidx, data, dcf, (uint32_t)C, ca, cbb,
+                                                 lane);
                 __builtin_amdgcn_sched_barrier(0);
-                apply_batch<NBLK, NCB, CT>(acc, S1, tile, slut, v0ok, v1ok, lane);
+                apply_batch<NBLK, NCB, CT, FULL>(acc, S1, tile, slut, v0ok, v1ok, r, r1, lane);
                 r += RB;
-                tb += T1;
                 if (r >= r1) break;
-                h1 = h3;
             }
         }
 
@@ -330,37 +355,47 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     }
 }
 
-template <int NBLK, int NCB, bool PAIR, bool CT, typename TS>
+template <int NBLK, int NCB, bool PAIR, bool CT, bool FULL, typename TS>
 fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out)
 {
     fr_ctx *ctx = p->ctx;
+    // the kernel addresses headers, tap starts, taps and DCFs by 32-bit byte offsets
+    const uint64_t lim = uint64_t(1) << 32;
+    FR_CHECK(ctx, (uint64_t)(p->nrecords + 64) * sizeof(uint2) < lim &&
+                      (uint64_t)(p->ntaps + 576) * (CT ? sizeof(uint16_t) : sizeof(uint2)) < lim &&
+                      (uint64_t)s->M * sizeof(float) < lim,
+             FR_ERR_UNSUPPORTED, "matrix-core gridding: plan of %lld records / %lld taps exceeds 4 GiB arrays "
+             "(use accumulate 2)", (long long)p->nrecords, (long long)p->ntaps);
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int chunks = (s->C + 8 * NCB - 1) / (8 * NCB);
     const int64_t light = p->nitems - p->nitems_front;
     const int64_t span = 8 * XCD_RUN;
     const int64_t W = p->nitems_front + span * ((light + span - 1) / span);
-    hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, CT, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
-                       p->items, p->nitems, p->units, p->hdrs, (const void *)p->taps, (const TS *)s->data, s->dcf,
+    hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, CT, FULL, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
+                       p->items, p->nitems, p->units, p->hdrs, p->tstart, (const void *)p->taps, (const TS *)s->data,
+                       s->dcf,
                        s->C, (int)p->N, cells, out, (float2 *)p->partial, p->nslots, p->geom.cells_max, p->lut,
-                       p->L, p->nitems_front);
+                       p->L, p->nitems_front, (uint32_t)p->ntaps);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
 
-template <int NBLK, int NCB, bool PAIR, typename TS>
+template <int NBLK, int NCB, bool PAIR, bool FULL, typename TS>
 fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out)
 {
-    return p->compact_taps ? launch_mfma_ct<NBLK, NCB, PAIR, true, TS>(p, s, out)
-                           : launch_mfma_ct<NBLK, NCB, PAIR, false, TS>(p, s, out);
+    return p->compact_taps ? launch_mfma_ct<NBLK, NCB, PAIR, true, FULL, TS>(p, s, out)
+                           : launch_mfma_ct<NBLK, NCB, PAIR, false, FULL, TS>(p, s, out);
 }
 
+// FULL: every chunk holds 8 NCB coils, so no lane's coil is masked
 template <int NBLK, typename TS>
 fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out)
 {
-    if (s->C <= 8) return launch_mfma<NBLK, 1, false, TS>(p, s, out);
-    if (s->C <= 16) return launch_mfma<NBLK, 2, false, TS>(p, s, out);
-    if (s->C % 2 == 0) return launch_mfma<NBLK, 4, true, TS>(p, s, out);
-    return launch_mfma<NBLK, 4, false, TS>(p, s, out);
+    if (s->C <= 8) return launch_mfma<NBLK, 1, false, false, TS>(p, s, out);
+    if (s->C <= 16) return launch_mfma<NBLK, 2, false, false, TS>(p, s, out);
+    if (s->C % 32 == 0) return launch_mfma<NBLK, 4, true, true, TS>(p, s, out);
+    if (s->C % 2 == 0) return launch_mfma<NBLK, 4, true, false, TS>(p, s, out);
+    return launch_mfma<NBLK, 4, false, false, TS>(p, s, out);
 }
 
 template <typename TS>

@@ -131,6 +131,7 @@ struct fr_plan {
     fr_unit *units = nullptr;          // device, heaviest first
     int64_t nunits = 0;
     uint2 *hdrs = nullptr;             // records: (sample, ntaps)
+    uint32_t *tstart = nullptr;        // records: index of the first tap (padding entries: the tap count)
     uint2 *taps = nullptr;             // taps: (LDS byte offset of the cell's row, LUT weight bits), or
                                        // uint16_t (cell | LUT index << 6) when compact_taps
     bool compact_taps = false;

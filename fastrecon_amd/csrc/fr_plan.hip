@@ -205,6 +205,22 @@ __global__ void k_fill_records(const float *__restrict__ traj, int64_t M, gctx g
     for (; i < np; ++i) w[i] = make_uint2(pad_x, 0u);
 }
 
+// the tap array's tail (past the last record's taps) holds padding taps
+template <typename TT>
+__global__ void k_pad_tail(TT *__restrict__ T, uint64_t from, int64_t n, TT pad)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) T[from + i] = pad;
+}
+
+// first tap of each record as 32 bits, and the tap count past the last record
+// (the headers' padding)
+__global__ void k_tap_starts(const uint64_t *__restrict__ toff, int64_t R, int64_t n, uint32_t *__restrict__ tstart)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) tstart[r] = (uint32_t)toff[r < R ? r : R];
+}
+
 // A record opens a new work item when it is its unit's first record or when
 // the unit's running tap count crosses a multiple of the slice size.
 __global__ void k_item_flags(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ toff, int64_t R,
@@ -685,6 +701,9 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     const size_t tap_bytes = p->compact_taps ? sizeof(uint16_t) : sizeof(uint2);
     const uint64_t tap_alloc = TT + 576;
     if ((e = fr_alloc(ctx, (void **)&p->hdrs, sizeof(uint2) * (R + 64)))) return e;
+    if ((e = fr_alloc(ctx, (void **)&p->tstart, sizeof(uint32_t) * (R + 64)))) return e;
+    hipLaunchKernelGGL(k_tap_starts, dim3(nblk(R + 64, 256)), dim3(256), 0, st, toff.p, (int64_t)R, (int64_t)R + 64,
+                       p->tstart);
     // padded so a wave may stage 512 taps from any 64-aligned chunk base
     if ((e = fr_alloc(ctx, (void **)&p->taps, tap_bytes * tap_alloc))) return e;
     FR_HIP(ctx, hipMemsetAsync(p->taps, 0, tap_bytes * tap_alloc, st));
@@ -698,6 +717,14 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     const uint32_t pad_x = gather ? 0xffffu
                                   : (p->compact_taps ? (uint32_t)geom.cells_max
                                                      : (uint32_t)geom.cells_max * 4u * FR_ACC_STRIDE);
+    // the matrix-core kernel reads entry TT for lanes without a tap
+    if (p->compact_taps)
+        hipLaunchKernelGGL(k_pad_tail<uint16_t>, dim3(nblk(tap_alloc - TT, 256)), dim3(256), 0, st,
+                           reinterpret_cast<uint16_t *>(p->taps), TT, (int64_t)(tap_alloc - TT),
+                           (uint16_t)(pad_x | ((uint32_t)p->L << 6)));
+    else if (!gather)
+        hipLaunchKernelGGL(k_pad_tail<uint2>, dim3(nblk(tap_alloc - TT, 256)), dim3(256), 0, st, p->taps, TT,
+                           (int64_t)(tap_alloc - TT), make_uint2(pad_x, 0u));
     if (R) {
         hipLaunchKernelGGL(k_unit_ranges, dim3(nblk(R, 256)), dim3(256), 0, st, keys_out.p, toff.p, (int64_t)R,
                            p->units);
@@ -860,7 +887,7 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
 {
     if (!p) return FR_OK;
     hipStreamSynchronize(p->ctx->stream);
-    hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps);
+    hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps); hipFree(p->tstart);
     hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches); hipFree(p->mfma_pairs);
     delete p;
     return FR_OK;
@@ -877,7 +904,7 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->slabs = p->geom.waves;
     info->tiles = (int)std::min<int64_t>(p->nunits, INT32_MAX);
     info->lds_bytes = 0;
-    info->plan_bytes = (int64_t)sizeof(uint2) * p->nrecords +
+    info->plan_bytes = (int64_t)(sizeof(uint2) + sizeof(uint32_t)) * p->nrecords +
                        (int64_t)(p->compact_taps ? sizeof(uint16_t) : sizeof(uint2)) * p->ntaps +
                        (int64_t)sizeof(fr_unit) * p->nunits;
     info->build_ms = p->build_ms;
