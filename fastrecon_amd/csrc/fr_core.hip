@@ -259,11 +259,33 @@ extern "C" fr_status fr_samples_set_traj(fr_samples *s, const float *traj, int s
     return FR_OK;
 }
 
+// A caller's device buffer must hold what the call reads: [p, p + bytes) has to
+// lie inside the allocation p points into.  Pointers the runtime cannot place
+// (e.g. imported or host memory) are not checked.
+static fr_status check_device_extent(fr_ctx *ctx, const void *p, size_t bytes, const char *what)
+{
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess || base == nullptr) {
+        (void)hipGetLastError();
+        return FR_OK;
+    }
+    const size_t avail = size - (size_t)((const char *)p - (const char *)base);
+    FR_CHECK(ctx, bytes <= avail, FR_ERR_INVALID, "%s: the call reads %zu bytes but the device allocation holds %zu "
+             "from this pointer", what, bytes, avail);
+    return FR_OK;
+}
+
 extern "C" fr_status fr_samples_set_traj_device(fr_samples *s, const float *traj, int stride)
 {
     if (!s || !traj) return FR_ERR_INVALID;
     fr_ctx *ctx = s->ctx;
     FR_CHECK(ctx, stride >= s->dim, FR_ERR_INVALID, "traj stride %d < dim %d", stride, s->dim);
+    if (s->M > 0) {
+        fr_status e = check_device_extent(ctx, traj, sizeof(float) * ((size_t)(s->M - 1) * stride + s->dim),
+                                          "fr_samples_set_traj_device");
+        if (e) return e;
+    }
     hipLaunchKernelGGL(k_deinterleave_traj, dim3(nblk(s->M, 256)), dim3(256), 0, ctx->stream,
                        traj, stride, s->dim, s->M, s->traj);
     FR_HIP(ctx, hipGetLastError());
@@ -285,6 +307,8 @@ extern "C" fr_status fr_samples_set_dcf_device(fr_samples *s, const float *dcf)
 {
     if (!s || !dcf) return FR_ERR_INVALID;
     fr_ctx *ctx = s->ctx;
+    fr_status e = check_device_extent(ctx, dcf, sizeof(float) * s->M, "fr_samples_set_dcf_device");
+    if (e) return e;
     FR_HIP(ctx, hipMemcpyAsync(s->dcf, dcf, sizeof(float) * s->M, hipMemcpyDeviceToDevice, ctx->stream));
     s->has_dcf = true;
     return FR_OK;
@@ -349,6 +373,9 @@ extern "C" fr_status fr_samples_set_data_device(fr_samples *s, const void *data,
     FR_CHECK(ctx, layout == FR_LAYOUT_COIL_MAJOR || layout == FR_LAYOUT_SAMPLE_MAJOR, FR_ERR_INVALID,
              "bad layout %d", layout);
     int64_t n = s->M * s->C;
+    // complex64 input whatever the sample type (FP16 samples are converted here)
+    fr_status e = check_device_extent(ctx, data, sizeof(float2) * (size_t)n, "fr_samples_set_data_device");
+    if (e) return e;
     if (layout == FR_LAYOUT_SAMPLE_MAJOR) {
         if (s->type == FR_SAMPLES_C64) {
             FR_HIP(ctx, hipMemcpyAsync(s->data, data, 8 * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));

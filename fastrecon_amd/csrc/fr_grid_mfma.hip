@@ -135,7 +135,11 @@ __device__ __forceinline__ void issue_batch(batch_st<NBLK, CT> &S, const hdr_st<
     for (int p = 0; p < 4 * NBLK; ++p) {
         const uint32_t rec = rb + p * RPP + lane / TPR;
         const uint32_t n = rec < r1 ? G.hp[p].y : 0u;
-        S.tp[p] = ld_off(T, (u < n ? G.t0[p] + u : padidx) * (uint32_t)sizeof(TT));
+        const uint32_t ti = u < n ? G.t0[p] + u : padidx;
+        if constexpr (CT)
+            S.tp[p] = ld_off(T, ti * (uint32_t)sizeof(TT));
+        else
+            S.tp[p] = T[ti];                    // 8-byte taps may pass 4 GiB
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -359,10 +363,10 @@ template <int NBLK, int NCB, bool PAIR, bool CT, bool FULL, typename TS>
 fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out)
 {
     fr_ctx *ctx = p->ctx;
-    // the kernel addresses headers, tap starts, taps and DCFs by 32-bit byte offsets
+    // the kernel addresses headers, tap starts, compact taps and DCFs by 32-bit byte offsets
     const uint64_t lim = uint64_t(1) << 32;
     FR_CHECK(ctx, (uint64_t)(p->nrecords + 64) * sizeof(uint2) < lim &&
-                      (uint64_t)(p->ntaps + 576) * (CT ? sizeof(uint16_t) : sizeof(uint2)) < lim &&
+                      (!CT || (uint64_t)(p->ntaps + 576) * sizeof(uint16_t) < lim) &&
                       (uint64_t)s->M * sizeof(float) < lim,
              FR_ERR_UNSUPPORTED, "matrix-core gridding: plan of %lld records / %lld taps exceeds 4 GiB arrays "
              "(use accumulate 2)", (long long)p->nrecords, (long long)p->ntaps);

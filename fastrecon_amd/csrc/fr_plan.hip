@@ -425,14 +425,26 @@ int floor_pow2(int v)
 // MI355X, 8 x 2 x 1 was fastest for the 32-coil 256^3 kooshball).
 static fr_geom choose_geom(int dim, unsigned N, int coils, int accumulate, const fr_tune *tune)
 {
-    (void)coils;
     fr_geom g{};
     g.dim = dim;
     g.N = (int)N;
     if (accumulate == FR_ACC_MFMA) {
-        g.bx = 8;
-        g.by = 2;
-        g.bz = 1;
+        // Default unit by coil count (matrix-core kernel, measured on the 3D
+        // kooshball at 256^3 and 384^3): with few coils a record's matrix
+        // work is small next to its fixed per-record cost, so bigger units
+        // (fewer records per sample) win; at 32 coils the smallest x-elongated
+        // unit does (1 coil: 4x4x2 2.6 ms vs 8x2x1 3.7; 8: 8x2x2 3.2 vs 4.1;
+        // 12: 8x2x2 3.7 vs 4.3; 32: 8x2x1 5.9 vs 8x2x2 7.1).  In 2D, 8x4 up
+        // to 16 coils (the 12-coil spiral series: 1.16 vs 1.36 ms).
+        if (dim == 3) {
+            g.bx = coils <= 2 ? 4 : 8;
+            g.by = coils <= 2 ? 4 : 2;
+            g.bz = coils <= 16 ? 2 : 1;
+        } else {
+            g.bx = 8;
+            g.by = coils <= 16 ? 4 : 2;
+            g.bz = 1;
+        }
     } else {
         g.bx = dim == 3 ? 4 : 8;
         g.by = 4;
