@@ -53,11 +53,17 @@ __host__ __device__ inline uint32_t f2bf(float f)
 
 __device__ inline float bf2f(uint32_t b) { return __uint_as_float(b << 16); }
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (v_rcp_f32, 1 ulp):
+// the network's BF16 weights and hidden layer set its precision, so an IEEE
+// division (about ten instructions, 32 per sample) buys nothing.  Training and
+// inference use the same functions.
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 
 __device__ inline float dsilu(float z)
 {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -241,7 +247,7 @@ __device__ inline void mlp_forward16(const WS &W, const u32x4 (&a)[4][2], const 
             const int k = 32 * ks + 8 * g + j;
             float z = W.b1(k);
 #pragma unroll
-            for (int i = 0; i < NF; ++i) z = __fadd_rn(z, __fmul_rn(x[i], W.w1(i, k)));
+            for (int i = 0; i < NF; ++i) z = fmaf(x[i], W.w1(i, k), z);
             F.z1[ks][j] = z;
             hb[j] = f2bf(silu(z));
             F.h1[ks][j] = bf2f(hb[j]);
@@ -262,7 +268,7 @@ __device__ inline void mlp_forward16(const WS &W, const u32x4 (&a)[4][2], const 
             const float z = __fadd_rn(acc[i], W.b2(o));
             F.z2[mt][i] = z;
             F.h2[mt][i] = silu(z);
-            part = __fadd_rn(part, __fmul_rn(F.h2[mt][i], W.w3(o)));
+            part = fmaf(F.h2[mt][i], W.w3(o), part);
         }
     }
     // the four lane groups hold the partial sums of the same sample
