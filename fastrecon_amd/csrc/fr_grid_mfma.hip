@@ -61,6 +61,22 @@ __device__ __forceinline__ float4 ld_pair(const __half2 *__restrict__ row, int c
 
 constexpr int RB = 16;                       // records per batch
 constexpr int64_t XCD_RUN = FR_MFMA_XCD_RUN; // consecutive items one XCD takes in turn
+// light items per workgroup, consecutive in an XCD's run: 4 for units of 32
+// or more cells (one LUT staging for four items measured 5% faster with 8x2x2
+// units), 1 for 16-cell units (4 per workgroup measured 6% slower there)
+template <int NBLK> constexpr int items_per_wg() { return NBLK >= 2 ? 4 : 1; }
+static_assert(XCD_RUN % 4 == 0, "XCD runs hold whole groups of items");
+
+// The workgroup is one wave.  LDS data written by some lanes and read by
+// others needs only the wave's in-order LDS queue and a compiler barrier, not
+// a workgroup barrier (whose fence also waits for every outstanding global
+// store, i.e. for the grid write-out).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 constexpr int TAP_ROW = 4 * FR_ACC_STRIDE;   // plan taps: (cell * TAP_ROW, LUT weight bits)
 // the pipeline reads headers up to 3 RB - 1 records past a live record; the
 // plan pads the headers and tap starts by 64 records
@@ -224,7 +240,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     __shared__ float slut[CT ? FR_CT_MAX_L + 1 : 1];
     if constexpr (CT) {
         for (int q = threadIdx.x; q <= L; q += 64) slut[q] = q < L ? lut[q] : 0.f;
-        __syncthreads();
+        wave_sync();
     }
     // LDS: the A tile during the record loop, then the write-out transpose
     constexpr int CS = 16 * NBLK + 1;           // cells per component row of the transpose
@@ -246,22 +262,31 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
     // inside the chunk, ca + 1 <= c0 + nc - 1 < C
     const int ca = PAIR ? c0 + min(cl0, nc - 2) : c0 + min(cl0, nc - 1), cbb = c0 + min(cl0 + 1, nc - 1);
 
-    // One workgroup per item, in the order of k_grid_items (fr_grid.hip):
-    // the heaviest units' items first, one per workgroup; the rest, in
-    // Morton order, remapped so that each XCD (workgroups are dealt to the 8
-    // XCDs round-robin) takes runs of XCD_RUN consecutive items.  The hardware
+    // Items in the order of k_grid_items (fr_grid.hip): the heaviest units'
+    // items first, one per workgroup; the rest, in Morton order, IPW
+    // consecutive items per workgroup (one LUT staging for all of them),
+    // remapped so that each XCD (workgroups are dealt to the 8 XCDs
+    // round-robin) takes runs of XCD_RUN consecutive items.  The hardware
     // dispatches workgroups in order, so each XCD's running items stay a
     // compact patch of neighbouring units that share samples in its L2 (a
     // persistent static schedule drifts apart and measured slower).
     const int64_t bid = blockIdx.x;
     int64_t i0;
+    int nown;
     if (bid < nfront) {
         i0 = bid;
+        nown = 1;
     } else {
+        constexpr int IPW = items_per_wg<NBLK>();
+        constexpr int64_t XR = XCD_RUN / IPW;   // groups per XCD run
         const int64_t rr = bid - nfront, gg = rr / 8;
-        i0 = nfront + (gg / XCD_RUN) * (8 * XCD_RUN) + (bid % 8) * XCD_RUN + gg % XCD_RUN;
+        i0 = nfront + ((gg / XR) * (8 * XR) + (bid % 8) * XR + gg % XR) * IPW;
+        nown = IPW;
     }
-    for (int64_t i = i0; i < nitems && i == i0; ++i) {
+#pragma unroll 1
+    for (int own = 0; own < nown; ++own) {
+        const int64_t i = i0 + own;
+        if (i >= nitems) break;
         const fr_item it = items[i];
         const fr_unit U = units[it.unit];
         const int ux = U.ext & 255, uy = (U.ext >> 8) & 255, uz = U.ext >> 16;
@@ -314,7 +339,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
         // 32 or 64 cells) every lane keeps one cell for the whole write-out
         // and steps through the coils; otherwise cell offsets go through LDS.
         const bool lane_cell = (64 % ncell) == 0;
-        __syncthreads();
+        wave_sync();
         if (!lane_cell) {
             for (int q = lane; q < ncell; q += 64) {
                 const int lx = q % ux, ly = (q / ux) % uy, lz = q / (ux * uy);
@@ -327,7 +352,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
             for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) osm[(NCB * j + cb) * CS + 16 * b + 4 * k + r] = acc[b][cb][r];
-        __syncthreads();
+        wave_sync();
         // each grid value is written once and not read again by this pass: a
         // streaming store keeps the sample rows in L2
         auto put = [&](int cl, int q, size_t go) {
@@ -355,7 +380,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
                 put(cl, q, goff[q]);
             }
         }
-        __syncthreads();
+        wave_sync();   // the next item reuses the LDS tile
     }
 }
 
@@ -373,8 +398,9 @@ fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out)
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int chunks = (s->C + 8 * NCB - 1) / (8 * NCB);
     const int64_t light = p->nitems - p->nitems_front;
-    const int64_t span = 8 * XCD_RUN;
-    const int64_t W = p->nitems_front + span * ((light + span - 1) / span);
+    constexpr int IPW = items_per_wg<NBLK>();
+    const int64_t groups = (light + IPW - 1) / IPW, span = 8 * (XCD_RUN / IPW);
+    const int64_t W = p->nitems_front + span * ((groups + span - 1) / span);
     hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, CT, FULL, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
                        p->items, p->nitems, p->units, p->hdrs, p->tstart, (const void *)p->taps, (const TS *)s->data,
                        s->dcf,

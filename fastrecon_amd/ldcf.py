@@ -195,8 +195,9 @@ def learned_dcf(ctx, mlp, samples, N):
 def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None, coils=None):
     """Learned vs iterative DCF on one acquisition (device resident after
     upload): end-to-end milliseconds of each recon from the plan (iterative:
-    Pipe-Menon + grid + FFT + SOS; learned: features + MLP + grid + FFT + SOS),
-    image NRMSE of learned (and ramp) against iterative, PSF sidelobes."""
+    Pipe-Menon + grid + FFT + SOS, on the recon's plan and on a second plan
+    shaped for single-channel gridding; learned: features + MLP + grid + FFT +
+    SOS), image NRMSE of learned (and ramp) against iterative, PSF sidelobes."""
     import torch  # device timing through HIP events on the context stream
 
     tn, _ = hip.normalize_traj(traj, N)
@@ -252,6 +253,21 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
 
     t_mr = timed(learned_refined)
     im_mr = img.to_host(np.float32)
+
+    # the iterative DCF on a second plan of the same samples with the unit
+    # shape that single-channel gridding prefers (the default shape for one
+    # coil), then the same recon: the fastest iterative path this library has
+    lut, _ = hip.kb_lut(KW, OG, L)
+    p1 = hip.Plan(ctx, s, N, KW, lut, dict(tile_x=4, tile_y=4, tile_z=2) if tn.shape[1] == 3
+                  else dict(tile_x=8, tile_y=4))
+
+    def iterative_own_plan():
+        hip.pipe_menon(p1, s, iters, True)
+        recon()
+
+    t_io = timed(iterative_own_plan)
+    im_io = img.to_host(np.float32)
+    p1.close()
     # one Pipe-Menon iteration (1 / kernel density) for reference
     hip.pipe_menon(p, s, 1, True)
     recon()
@@ -260,6 +276,8 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
                nrmse_pm1=nrmse(im_pm1, im_it), nrmse_pm1_scaled=nrmse_scaled(im_pm1, im_it),
                nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
                ms_learned_pm1=t_mr, speedup_learned_pm1=t_it / t_mr,
+               ms_iterative_own_plan=t_io, speedup_vs_iterative_own_plan=t_io / t_ml,
+               nrmse_iterative_own_plan=nrmse(im_io, im_it),
                nrmse_learned_pm1=nrmse(im_mr, im_it), nrmse_learned_pm1_scaled=nrmse_scaled(im_mr, im_it),
                dcf_rel_rms=float(np.sqrt(np.mean((w_ml / np.maximum(w_it, 1e-30) - 1.0) ** 2))))
     if ramp is not None:
