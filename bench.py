@@ -186,6 +186,46 @@ def bench_fp16(ctx, torch, hip, syn, steps):
     return res
 
 
+def bench_radial2d(ctx, torch, hip, syn, steps):
+    """2D golden-angle radial, 402 spokes x 512 samples, 8 coils, 256^2 image,
+    og 2 -> 512^2 grid (the reference program's CPU-sized case): gridding and
+    the recon with the cropped transforms, shift, crop and coil sum."""
+    C, image = 8, 256
+    N = syn.grid_size(image, 2.0)
+    traj, dcf = syn.radial2d(402, 512)
+    M = traj.shape[0]
+    tn, _ = hip.normalize_traj(traj, N)
+    s = hip.Samples(ctx, 2, M, C)
+    s.set_traj(tn)
+    s.set_dcf(dcf)
+    device_data(torch, ctx, s, M, C, 4)
+    lut, _ = hip.kb_lut(4.0, 2.0, 512)
+    t0 = time.perf_counter()
+    p = hip.Plan(ctx, s, N, 4.0, lut)
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    g = hip.Grids(ctx, 2, N, C)
+    f = hip.FFT(ctx, 2, N, C, +1)
+    img = hip.DeviceBuffer(ctx, 4 * image * image)
+    rc = (image, image, 1)
+    hip.grid_execute(p, s, g)
+    hip.fft_sos(f, g, rc, True, None, img)
+    grid_ms = time_stream(torch, ctx, lambda: hip.grid_execute(p, s, g), steps)
+
+    def recon():
+        hip.grid_execute(p, s, g)
+        hip.fft_sos(f, g, rc, True, None, img)
+
+    recon_ms = time_stream(torch, ctx, recon, steps)
+    b = C * M * 8 + M * 12 + C * N * N * 8
+    res = {"workload": f"radial2d 402 spokes x 512, {C} coils, {image}^2 image, og 2 -> {N}^2 grid",
+           "samples_per_coil": M, "plan_ms": plan_ms, "grid_ms": grid_ms, "recon_ms": recon_ms,
+           "gridded_coil_samples_per_s": C * M / (grid_ms * 1e-3), "algorithmic_bytes": b,
+           "achieved_gbs": b / (grid_ms * 1e-3) / 1e9}
+    for h in (img, f, g, p, s):
+        h.close()
+    return res
+
+
 def cpu_baseline(traj_n, dcf, N, coils, lut, target_s):
     """Oracle FP32 restatement of GridLut::griddingChannel on a bounded
     slice (the first spokes of the same trajectory), OpenMP over coils with
@@ -359,6 +399,8 @@ def main():
         grids = hip.Grids(ctx, 3, N, C)
     if not a.no_fp16:
         out["fp16_kooshball3d"] = bench_fp16(ctx, torch, hip, syn, max(3, a.steps // 4))
+    if not a.no_frames:
+        out["radial2d"] = bench_radial2d(ctx, torch, hip, syn, a.steps)
 
     if not a.no_ldcf:
         # learned vs iterative DCF on this workload's geometry (the network's
