@@ -259,11 +259,20 @@ extern "C" fr_status fr_samples_set_traj(fr_samples *s, const float *traj, int s
     return FR_OK;
 }
 
-// A caller's device buffer must hold what the call reads: [p, p + bytes) has to
-// lie inside the allocation p points into.  Pointers the runtime cannot place
-// (e.g. imported or host memory) are not checked.
-static fr_status check_device_extent(fr_ctx *ctx, const void *p, size_t bytes, const char *what)
+// A caller's device buffer must hold what the call reads or writes: [p, p +
+// bytes) has to lie inside the allocation p points into.  Pointers the
+// runtime cannot place (e.g. imported or host memory) are not checked, nor
+// are calls made while the context's stream is being captured (the eager
+// pass before a capture checks them).
+fr_status fr_check_device_extent(fr_ctx *ctx, const void *p, size_t bytes, const char *what)
 {
+    if (!p || bytes == 0) return FR_OK;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess) {
+        (void)hipGetLastError();
+        return FR_OK;
+    }
+    if (cap != hipStreamCaptureStatusNone) return FR_OK;
     hipDeviceptr_t base = nullptr;
     size_t size = 0;
     if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess || base == nullptr) {
@@ -271,8 +280,8 @@ static fr_status check_device_extent(fr_ctx *ctx, const void *p, size_t bytes, c
         return FR_OK;
     }
     const size_t avail = size - (size_t)((const char *)p - (const char *)base);
-    FR_CHECK(ctx, bytes <= avail, FR_ERR_INVALID, "%s: the call reads %zu bytes but the device allocation holds %zu "
-             "from this pointer", what, bytes, avail);
+    FR_CHECK(ctx, bytes <= avail, FR_ERR_INVALID, "%s: the call accesses %zu bytes but the device allocation holds "
+             "%zu from this pointer", what, bytes, avail);
     return FR_OK;
 }
 
@@ -282,7 +291,7 @@ extern "C" fr_status fr_samples_set_traj_device(fr_samples *s, const float *traj
     fr_ctx *ctx = s->ctx;
     FR_CHECK(ctx, stride >= s->dim, FR_ERR_INVALID, "traj stride %d < dim %d", stride, s->dim);
     if (s->M > 0) {
-        fr_status e = check_device_extent(ctx, traj, sizeof(float) * ((size_t)(s->M - 1) * stride + s->dim),
+        fr_status e = fr_check_device_extent(ctx, traj, sizeof(float) * ((size_t)(s->M - 1) * stride + s->dim),
                                           "fr_samples_set_traj_device");
         if (e) return e;
     }
@@ -307,7 +316,7 @@ extern "C" fr_status fr_samples_set_dcf_device(fr_samples *s, const float *dcf)
 {
     if (!s || !dcf) return FR_ERR_INVALID;
     fr_ctx *ctx = s->ctx;
-    fr_status e = check_device_extent(ctx, dcf, sizeof(float) * s->M, "fr_samples_set_dcf_device");
+    fr_status e = fr_check_device_extent(ctx, dcf, sizeof(float) * s->M, "fr_samples_set_dcf_device");
     if (e) return e;
     FR_HIP(ctx, hipMemcpyAsync(s->dcf, dcf, sizeof(float) * s->M, hipMemcpyDeviceToDevice, ctx->stream));
     s->has_dcf = true;
@@ -374,7 +383,7 @@ extern "C" fr_status fr_samples_set_data_device(fr_samples *s, const void *data,
              "bad layout %d", layout);
     int64_t n = s->M * s->C;
     // complex64 input whatever the sample type (FP16 samples are converted here)
-    fr_status e = check_device_extent(ctx, data, sizeof(float2) * (size_t)n, "fr_samples_set_data_device");
+    fr_status e = fr_check_device_extent(ctx, data, sizeof(float2) * (size_t)n, "fr_samples_set_data_device");
     if (e) return e;
     if (layout == FR_LAYOUT_SAMPLE_MAJOR) {
         if (s->type == FR_SAMPLES_C64) {

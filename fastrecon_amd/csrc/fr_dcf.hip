@@ -132,6 +132,7 @@ extern "C" fr_status fr_degrid_execute(fr_plan *p, const fr_samples *s, const fr
     FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID, "samples do not match the plan");
     FR_CHECK(ctx, g->dim == p->dim && g->N == p->N && g->frames == p->frames && s->frames == p->frames,
              FR_ERR_INVALID, "grids do not match the plan");
+    FR_EXTENT(ctx, out_dev, sizeof(float2) * (size_t)s->M * g->C, "fr_degrid_execute output");
     return fr_degrid_raw(p, s, g->data, g->C, (float2 *)out_dev);
 }
 

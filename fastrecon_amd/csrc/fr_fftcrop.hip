@@ -303,6 +303,8 @@ bool fr_fft_sos_fused(const fr_fft *f, const fr_grids *g, unsigned rcx, unsigned
            (g->dim == 2 || rcz >= 1) && (int64_t)g->C * std::max(g->frames, 1) == f->batch;
 }
 
+// Twiddles exp(sign 2 pi i k / N), k < N, FP64-evaluated, uploaded once per
+// plan (the first fr_fft_sos call; run it before capturing a graph).
 fr_status fr_fft_tw(fr_fft *f)
 {
     if (f->tw) return FR_OK;
@@ -311,10 +313,16 @@ fr_status fr_fft_tw(fr_fft *f)
         const double a = 2.0 * M_PI * (double)k / (double)f->N * (double)f->sign;
         tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
-    fr_status e = fr_alloc(f->ctx, (void **)&f->tw, sizeof(float2) * f->N);
+    float2 *d = nullptr;
+    fr_status e = fr_alloc(f->ctx, (void **)&d, sizeof(float2) * f->N);
     if (e) return e;
-    FR_HIP(f->ctx, hipMemcpyAsync(f->tw, tw.data(), sizeof(float2) * f->N, hipMemcpyHostToDevice, f->ctx->stream));
-    FR_HIP(f->ctx, hipStreamSynchronize(f->ctx->stream));
+    hipError_t h = hipMemcpyAsync(d, tw.data(), sizeof(float2) * f->N, hipMemcpyHostToDevice, f->ctx->stream);
+    if (h == hipSuccess) h = hipStreamSynchronize(f->ctx->stream);
+    if (h != hipSuccess) {
+        (void)hipFree(d);
+        return fr_fail(f->ctx, FR_ERR_HIP, "twiddle upload: %s", hipGetErrorString(h));
+    }
+    f->tw = d;
     return FR_OK;
 }
 
@@ -330,6 +338,8 @@ extern "C" fr_status fr_fft_sos(fr_fft *f, fr_grids *g, unsigned rcx, unsigned r
     FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
              FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
     FR_CHECK(ctx, !apply_shift || g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size");
+    FR_EXTENT(ctx, out_dev, sizeof(float) * rcx * rcy * rcz, "fr_fft_sos image");
+    FR_EXTENT(ctx, deapod_dev, sizeof(float) * rcx * rcy * (g->dim == 3 ? rcz : 1u), "fr_fft_sos deapodisation map");
     if (!fr_fft_sos_fused(f, g, rcx, rcy, rcz)) {
         // other sizes: the full batched FFT in place, then the fused epilogue
         fr_status e = fr_fft_execute(f, g);

@@ -160,6 +160,8 @@ extern "C" fr_status fr_sos(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsign
              FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
     FR_CHECK(ctx, !apply_shift || g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size");
     const size_t n = (size_t)rcx * rcy * rcz;
+    FR_EXTENT(ctx, out_dev, sizeof(float) * n, "fr_sos image");
+    FR_EXTENT(ctx, deapod_dev, sizeof(float) * rcx * rcy * (g->dim == 3 ? rcz : 1u), "fr_sos deapodisation map");
     hipLaunchKernelGGL(k_sos, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N, g->C,
                        g->cells, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev, g->frames);
     FR_HIP(ctx, hipGetLastError());
@@ -333,6 +335,7 @@ extern "C" fr_status fr_crop(fr_ctx *ctx, const fr_grids *g, unsigned rcx, unsig
     FR_CHECK(ctx, rcx >= 1 && rcy >= 1 && rcz >= 1 && rcx <= g->N && rcy <= g->N && (g->dim == 2 || rcz <= g->N),
              FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
     const size_t n = (size_t)rcx * rcy * rcz * g->C;
+    FR_EXTENT(ctx, out_dev, sizeof(float2) * n, "fr_crop output");
     hipLaunchKernelGGL(k_crop, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, g->dim, (int)g->N, g->C,
                        g->cells, (int)rcx, (int)rcy, (int)rcz, (float2 *)out_dev);
     FR_HIP(ctx, hipGetLastError());
@@ -351,6 +354,7 @@ extern "C" fr_status fr_sos_map(fr_ctx *ctx, const fr_grids *g, const fr_grids *
              FR_ERR_INVALID, "crop %ux%ux%u outside grid %u", rcx, rcy, rcz, g->N);
     FR_CHECK(ctx, !apply_shift || g->N % 2 == 0, FR_ERR_UNSUPPORTED, "fftshift needs an even grid size");
     const size_t n = (size_t)rcx * rcy * rcz;
+    FR_EXTENT(ctx, out_dev, sizeof(float2) * n, "fr_sos_map output");
     hipLaunchKernelGGL(k_sos_map, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, g->data, map->data, g->dim,
                        (int)g->N, g->C, g->cells, (int)rcx, (int)rcy, (int)rcz, apply_shift, (float2 *)out_dev);
     FR_HIP(ctx, hipGetLastError());

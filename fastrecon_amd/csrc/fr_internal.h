@@ -209,6 +209,13 @@ fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force
 fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out);
 // cropped FFT + SOS helpers (fr_fftcrop.hip)
 fr_status fr_fft_workspace(fr_fft *f, size_t bytes);
+// FR_ERR_INVALID unless [p, p + bytes) lies inside the device allocation of p
+fr_status fr_check_device_extent(fr_ctx *ctx, const void *p, size_t bytes, const char *what);
+#define FR_EXTENT(ctx, p, bytes, what)                                          \
+    do {                                                                        \
+        fr_status fr_extent_st_ = fr_check_device_extent((ctx), (p), (bytes), (what)); \
+        if (fr_extent_st_) return fr_extent_st_;                                \
+    } while (0)
 fr_status fr_fft_tw(fr_fft *f);
 bool fr_fft_sos_fused(const fr_fft *f, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz);
 fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune);

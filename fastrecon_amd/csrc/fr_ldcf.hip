@@ -545,6 +545,7 @@ extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned
     FR_CHECK(ctx, s->has_traj, FR_ERR_STATE, "trajectory not set");
     FR_CHECK(ctx, s->frames == 0, FR_ERR_UNSUPPORTED, "features of batched frames: compute them per frame");
     FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
+    FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_ldcf_features rows");
     const size_t cells = fr_cells(s->dim, N);
     uint32_t *hist = nullptr;
     fr_status st = fr_alloc(ctx, (void **)&hist, cells * sizeof(uint32_t));
@@ -635,6 +636,8 @@ extern "C" fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M,
     if (!m || !feat_dev || !y_dev || M < 0) return FR_ERR_INVALID;
     fr_ctx *ctx = m->ctx;
     if (M == 0) return FR_OK;
+    FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)M, "fr_mlp_forward feature rows");
+    FR_EXTENT(ctx, y_dev, sizeof(float) * (size_t)M, "fr_mlp_forward output");
     hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, M)), dim3(256), 0, ctx->stream, view(m), feat_dev, M, 0,
                        0.f, 1.f, y_dev);
     FR_HIP(ctx, hipGetLastError());
@@ -675,6 +678,7 @@ extern "C" fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s)
 {
     if (!m || !feat_dev || !s) return FR_ERR_INVALID;
     fr_ctx *ctx = m->ctx;
+    FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_mlp_dcf feature rows");
     if (s->M > 0)
         hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, s->M)), dim3(256), 0, ctx->stream, view(m), feat_dev,
                            s->M, 1, m->mu, m->sigma, s->dcf);
