@@ -68,17 +68,55 @@ __device__ inline float dsilu(float z)
 }
 
 // ------------------------------------------------------------- features
-__global__ void k_cell_hist(const float *__restrict__ traj, int64_t M, int dim, int N, uint32_t *__restrict__ hist)
+// Cell histogram of the samples' nearest cells.  Radial-type trajectories
+// cross the k-space centre once per spoke, so the few cells around N/2 take
+// thousands of hits that serialise as same-address atomics; each workgroup
+// counts the box of HR cells around the centre in LDS over its samples
+// (grid-stride) and adds the box to memory once.  Integer counts: the
+// result does not depend on order.
+constexpr int HR = 3;                        // central box half-width (cells)
+constexpr int HB = 2 * HR + 1;
+
+__global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ traj, int64_t M, int dim, int N,
+                                                   uint32_t *__restrict__ hist)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    size_t cell = 0;
-    for (int d = dim - 1; d >= 0; --d) {
-        int c = (int)floorf(traj[d * M + s] + 0.5f);
-        c = min(max(c, 0), N - 1);
-        cell = cell * N + c;
+    __shared__ uint32_t hot[HB * HB * HB];
+    const int nhot = dim == 3 ? HB * HB * HB : HB * HB;
+    for (int i = threadIdx.x; i < nhot; i += blockDim.x) hot[i] = 0u;
+    __syncthreads();
+    const int lo = N / 2 - HR;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < M; s += stride) {
+        size_t cell = 0;
+        int hidx = 0;
+        bool central = true;
+        for (int d = dim - 1; d >= 0; --d) {
+            int c = (int)floorf(traj[d * M + s] + 0.5f);
+            c = min(max(c, 0), N - 1);
+            cell = cell * N + c;
+            const int h = c - lo;
+            central = central && h >= 0 && h < HB;
+            hidx = hidx * HB + h;
+        }
+        if (central)
+            atomicAdd(&hot[hidx], 1u);
+        else
+            atomicAdd(&hist[cell], 1u);
     }
-    atomicAdd(&hist[cell], 1u);   // integer counts: the result does not depend on order
+    __syncthreads();
+    for (int i = threadIdx.x; i < nhot; i += blockDim.x) {
+        const uint32_t v = hot[i];
+        if (v == 0u) continue;
+        size_t cell = 0;
+        int r = i;
+        int c[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            c[d] = lo + r % HB;
+            r /= HB;
+        }
+        for (int d = dim - 1; d >= 0; --d) cell = cell * N + c[d];
+        atomicAdd(&hist[cell], v);
+    }
 }
 
 // inclusive prefix sum along one axis; axis stride `st`, lines enumerated by
@@ -98,10 +136,25 @@ __global__ void k_prefix_axis(uint32_t *__restrict__ a, int N, int dim, int axis
         else if (axis == 1) { base = (size_t)q * N * N + p; st = N; }
         else { base = (size_t)q * N + p; st = (size_t)N * N; }
     }
+    // 16 loads in flight, then their running sums (a store to a[] may alias a
+    // later load, so one load and one store per step would serialise on the
+    // memory latency)
+    constexpr int K = 16;
     uint32_t run = 0;
-    for (int i = 0; i < N; ++i) {
-        run += a[base + i * st];
-        a[base + i * st] = run;
+    int i = 0;
+    for (; i + K <= N; i += K) {
+        uint32_t v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = a[base + (size_t)(i + k) * st];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            run += v[k];
+            a[base + (size_t)(i + k) * st] = run;
+        }
+    }
+    for (; i < N; ++i) {
+        run += a[base + (size_t)i * st];
+        a[base + (size_t)i * st] = run;
     }
 }
 
@@ -533,7 +586,10 @@ extern "C" int fr_mlp_param_count(void) { return NPARAM; }
 // zeroed histogram -> per-cell counts -> summed-area table (in place)
 static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32_t *hist)
 {
-    hipLaunchKernelGGL(k_cell_hist, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist);
+    // about 32 samples per thread: few enough workgroups that the central
+    // box is flushed a few thousand times, not once per spoke
+    const unsigned wgs = std::max(1u, std::min(nblk(s->M, 256), nblk(s->M, 256 * 32)));
+    hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist);
     const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
     for (int ax = 0; ax < s->dim; ++ax)
         hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, hist, (int)N, s->dim, ax);
