@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recon", action="store_true")
     ap.add_argument("--no-ldcf", action="store_true", help="skip the learned-vs-iterative DCF leg")
-    ap.add_argument("--no-frames", action="store_true", help="skip the batched 2D spiral frames leg")
+    ap.add_argument("--no-frames", action="store_true", help="skip the batched 2D spiral frames leg and the 2D radial leg")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-sample 384^3 leg")
     ap.add_argument("--accumulate", type=int, default=0)
     ap.add_argument("--exact", action="store_true", help="exact-order plan for the timed steps")

@@ -26,14 +26,16 @@
 
 namespace {
 
-constexpr int TR = 16;          // rows per workgroup tile
+constexpr int TR = 16;          // rows per workgroup tile of the x and y passes
+constexpr int TS = 8;           // rows (output columns) per workgroup tile of the last pass: twice the
+                                // workgroups of a 16-row tile (the pass has only rc^2 / TS tiles)
 // waves per workgroup, each taking TR / NW rows of the tile (8 waves for
 // N = 256 measured slower than 4: 2.26 vs 2.16 ms for 32 coils at 256^3)
 #ifndef FR_FFT_WAVES
 #define FR_FFT_WAVES 4
 #endif
 template <int N> constexpr int waves() { return N >= 512 ? 4 : FR_FFT_WAVES; }
-static_assert(TR % (2 * FR_FFT_WAVES) == 0 && TR % 8 == 0,
+static_assert(TR % (2 * FR_FFT_WAVES) == 0 && TS % (2 * FR_FFT_WAVES) == 0 && 16 % TS == 0,
               "an even number of rows per wave (the prefetch alternates two buffers)");
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
@@ -169,10 +171,10 @@ __global__ void __launch_bounds__(WGT) k_fft_pass(const float2 *__restrict__ in,
     }
 }
 
-// Last pass with the coil sum: tile (a = outer slice, tt = TR rows), coils
-// c = 0 .. C-1 in order; row t of coil c at in + c inC + a inA + (tt TR + t)
+// Last pass with the coil sum: tile (a = outer slice, tt = TS rows), coils
+// c = 0 .. C-1 in order; row t of coil c at in + c inC + a inA + (tt TS + t)
 // rowStride.  acc[k'][t] += fl(re^2) + fl(im^2); then sqrt, deapod, store to
-// img[a outA + k' outK + tt TR + t].
+// img[a outA + k' outK + tt TS + t].
 template <int N, int NW = waves<N>(), int WGT = 64 * waves<N>()>
 __global__ void __launch_bounds__(WGT) k_fft_sos(const float2 *__restrict__ in, float *__restrict__ img,
                                                  const float2 *__restrict__ gtw, int64_t na, int64_t nt, int C,
@@ -182,26 +184,26 @@ __global__ void __launch_bounds__(WGT) k_fft_sos(const float2 *__restrict__ in, 
 {
     __shared__ float2 tw[N];
     __shared__ float2 fbuf[NW][2 * N];
-    extern __shared__ float acc[];               // [rc][TR]
+    extern __shared__ float acc[];               // [rc][TS]
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < N; i += WGT) tw[i] = gtw[i];
-    for (int i = threadIdx.x; i < rc * TR; i += WGT) acc[i] = 0.f;
+    for (int i = threadIdx.x; i < rc * TS; i += WGT) acc[i] = 0.f;
     __syncthreads();
     const int64_t T = blockIdx.x;
     const int64_t tt = T % nt, a = T / nt;
     // rows (c, w + NW i) in coil order, each wave's next row loaded ahead
-    const float2 *tbase = in + a * inA + tt * TR * rowStride;
+    const float2 *tbase = in + a * inA + tt * TS * rowStride;
     float2 xa[N / 256][4], xb[N / 256][4];
     load_row<N>(tbase + (int64_t)w * rowStride, xa, lane);
 #pragma unroll 1
     for (int c = 0; c < C; ++c) {
 #pragma unroll
-        for (int i = 0; i < TR / NW; ++i) {
+        for (int i = 0; i < TS / NW; ++i) {
             const int t = w + NW * i;
             float2 (&cur)[N / 256][4] = (i & 1) ? xb : xa;
             float2 (&nxt)[N / 256][4] = (i & 1) ? xa : xb;
             // next row: (c, t + NW), or (c + 1, w) after the last of this coil
-            if (i + 1 < TR / NW)
+            if (i + 1 < TS / NW)
                 load_row<N>(tbase + c * inC + (int64_t)(t + NW) * rowStride, nxt, lane);
             else if (c + 1 < C)
                 load_row<N>(tbase + (c + 1) * inC + (int64_t)w * rowStride, nxt, lane);
@@ -211,16 +213,16 @@ __global__ void __launch_bounds__(WGT) k_fft_sos(const float2 *__restrict__ in, 
             for (int q = 0; q < N / 64; ++q) {
                 const int kp = crop_pos(lane + 64 * q, N, rc, shift);
                 if (kp >= 0) {
-                    float &x = acc[kp * TR + t];
+                    float &x = acc[kp * TS + t];
                     x = __fadd_rn(x, __fadd_rn(__fmul_rn(v[q].x, v[q].x), __fmul_rn(v[q].y, v[q].y)));
                 }
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < rc * TR; i += WGT) {
-        const int kp = i / TR, t = i % TR;
-        const int64_t o = a * outA + (int64_t)kp * outK + tt * TR + t;
+    for (int i = threadIdx.x; i < rc * TS; i += WGT) {
+        const int kp = i / TS, t = i % TS;
+        const int64_t o = a * outA + (int64_t)kp * outK + tt * TS + t;
         float m = sqrtf(acc[i]);
         if (deapod) m = __fdiv_rn(m, deapod[o % dmod]);
         img[o] = m;
@@ -255,8 +257,8 @@ fr_status run_passes(fr_fft *f, const fr_grids *g, int rcx, int rcy, int rcz, in
         hipLaunchKernelGGL(k_fft_pass<N>, dim3((unsigned)(C * rcx * (N / TR))), dim3(WGT), lds_row * rcy,
                            ctx->stream, B1, B2, f->tw, d2, rcy, shift, s);
         // z + coil sum: tiles (ky, kx-block); row (c, ky, kx) of B2; img[kz][ky][kx]
-        hipLaunchKernelGGL(k_fft_sos<N>, dim3((unsigned)(rcy * (rcx / TR))), dim3(WGT), sizeof(float) * 16 * rcz,
-                           ctx->stream, B2, out, f->tw, (int64_t)rcy, (int64_t)(rcx / TR), C,
+        hipLaunchKernelGGL(k_fft_sos<N>, dim3((unsigned)(rcy * (rcx / TS))), dim3(WGT), sizeof(float) * TS * rcz,
+                           ctx->stream, B2, out, f->tw, (int64_t)rcy, (int64_t)(rcx / TS), C,
                            (int64_t)rcy * rcx * N, (int64_t)rcx * N, (int64_t)N, (int64_t)rcx,
                            (int64_t)rcy * rcx, rcz, shift, s, deapod, (int64_t)rcz * rcy * rcx);
     } else {
@@ -272,8 +274,8 @@ fr_status run_passes(fr_fft *f, const fr_grids *g, int rcx, int rcy, int rcz, in
                            g->data, B1, f->tw, d1, rcx, shift, s);
         // y + coil sum: tiles (f, kx-block); row (c, f, kx) of B1; img[f][ky][kx]
         // (the deapodisation map is one rcy x rcx image shared by the frames)
-        hipLaunchKernelGGL(k_fft_sos<N>, dim3((unsigned)(F * (rcx / TR))), dim3(WGT), sizeof(float) * 16 * rcy,
-                           ctx->stream, B1, out, f->tw, (int64_t)F, (int64_t)(rcx / TR), C, (int64_t)F * rcx * N,
+        hipLaunchKernelGGL(k_fft_sos<N>, dim3((unsigned)(F * (rcx / TS))), dim3(WGT), sizeof(float) * TS * rcy,
+                           ctx->stream, B1, out, f->tw, (int64_t)F, (int64_t)(rcx / TS), C, (int64_t)F * rcx * N,
                            (int64_t)rcx * N, (int64_t)N, (int64_t)rcy * rcx, (int64_t)rcx, rcy, shift, s, deapod,
                            (int64_t)rcy * rcx);
     }
