@@ -32,8 +32,11 @@ def samples_for(ctx, tn):
     return s
 
 
-@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (2, 48, 2500)])
+@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (2, 48, 2500), (3, 5, 400), (2, 6, 300)])
 def test_features_match_oracle(ctx, dim, N, M):
+    """Half the points gathered around the grid centre (the cells every
+    radial spoke crosses, counted per workgroup in LDS), half uniform; N = 5
+    and 6 are smaller than that central box."""
     rng = np.random.default_rng(dim + 10)
     t = np.concatenate([rng.normal(N / 2, 2.5, (M // 2, dim)), rng.uniform(-0.7, N - 0.3, (M - M // 2, dim))])
     t = t.astype(np.float32)
@@ -49,7 +52,7 @@ def test_features_match_oracle(ctx, dim, N, M):
     assert np.all(f[:, 6:] == 0)
 
 
-@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41)])
+@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41), (3, 6, 500)])
 def test_fused_dcf_equals_features_then_network(ctx, dim, N, M):
     """fr_ldcf_dcf (features in LDS, no feature rows) is bitwise the DCF of
     fr_ldcf_features + fr_mlp_dcf; M below, at and off multiples of 64, and a
