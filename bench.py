@@ -158,7 +158,8 @@ def bench_frames(ctx, torch, hip, syn, steps):
 
 def bench_fp16(ctx, torch, hip, syn, steps):
     """3D kooshball 100,000 spokes x 384, 16 coils, samples stored as FP16
-    pairs (FP32 accumulation), 256^3 image, og 1.5 -> 384^3 grid."""
+    pairs (FP32 accumulation), 256^3 image, og 1.5 -> 384^3 grid: gridding,
+    and the recon with the cropped N = 384 transforms, shift and coil sum."""
     C, image, og = 16, 256, 1.5
     N = syn.grid_size(image, og)
     traj, dcf = syn.kooshball3d(100000, 384)
@@ -174,14 +175,23 @@ def bench_fp16(ctx, torch, hip, syn, steps):
     p = hip.Plan(ctx, s, N, 4.0, lut)
     plan_ms = (time.perf_counter() - t0) * 1e3
     g = hip.Grids(ctx, 3, N, C)
+    f = hip.FFT(ctx, 3, N, C, +1)
+    img = hip.DeviceBuffer(ctx, 4 * image ** 3)
+    rc = (image, image, image)
     hip.grid_execute(p, s, g)
+    hip.fft_sos(f, g, rc, True, None, img)
     ms = time_stream(torch, ctx, lambda: hip.grid_execute(p, s, g), steps)
+
+    def recon():
+        hip.grid_execute(p, s, g)
+        hip.fft_sos(f, g, rc, True, None, img)
+    recon_ms = time_stream(torch, ctx, recon, steps)
     b = C * M * 4 + M * 16 + C * N ** 3 * 8
     res = {"workload": f"kooshball3d 100000 spokes x 384, {C} coils FP16, {image}^3 image, og 1.5 -> {N}^3 grid",
-           "samples_per_coil": M, "plan_ms": plan_ms, "grid_ms": ms,
+           "samples_per_coil": M, "plan_ms": plan_ms, "grid_ms": ms, "recon_ms": recon_ms,
            "gridded_coil_samples_per_s": C * M / (ms * 1e-3), "algorithmic_bytes": b,
            "achieved_gbs": b / (ms * 1e-3) / 1e9, "frac_of_8tbs": b / (ms * 1e-3) / 8e12}
-    for h in (g, p, s):
+    for h in (img, f, g, p, s):
         h.close()
     return res
 

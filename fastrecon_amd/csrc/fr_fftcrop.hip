@@ -16,7 +16,7 @@
 // optional deapodisation divide: the arithmetic of k_sos (fr_image.hip).
 //
 // Row transform: N-point Stockham radix-4 (N = 4^m, or 2 * 4^m with a final
-// radix-2 step), one wave per row, twiddles exp(+2 pi i k / N) from an FP64
+// radix-2 step; N = 384 takes radices 4, 4, 4, 2, 3), one wave per row, twiddles exp(+2 pi i k / N) from an FP64
 // table rounded to FP32.  The result differs from rocFFT's in rounding
 // (relative ~1e-7), well inside the 1e-5 image bar.
 #include "fr_internal.h"
@@ -46,70 +46,147 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y)); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y)); }
 
+// radix-4 butterflies per lane in one iteration (N / 4 of them; for N = 384
+// the second set covers lanes 0 .. 31 only)
+template <int N> constexpr int b4() { return (N / 4 + 63) / 64; }
+
 // The first radix-4 iteration's inputs of the row at `in` (unit stride):
 // lane l, butterfly b takes elements j + r N/4, j = l + 64 b.  Loaded one row
 // ahead so that the global-memory latency overlaps the previous row's work.
 template <int N>
-__device__ __forceinline__ void load_row(const float2 *__restrict__ in, float2 (&x0)[N / 256][4], int lane)
+__device__ __forceinline__ void load_row(const float2 *__restrict__ in, float2 (&x0)[b4<N>()][4], int lane)
 {
 #pragma unroll
-    for (int b = 0; b < N / 256; ++b)
+    for (int b = 0; b < b4<N>(); ++b) {
+        const int j = lane + 64 * b;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) x0[b][r] = in[lane + 64 * b + r * (N / 4)];
+        for (int r = 0; r < 4; ++r) x0[b][r] = j < N / 4 ? in[j + r * (N / 4)] : make_float2(0.f, 0.f);
+    }
 }
 
 // Natural-order N-point transform of a row whose first-iteration inputs are
 // in x0 (load_row).  After the call, lane l holds output k = l + 64 q in v[q]
 // (q = 0 .. N/64 - 1).  buf: 2N float2 of this wave's LDS; tw: N twiddles
 // exp(s 2 pi i k / N).
+// One Stockham radix-4 iteration over stride Ns (the product of the radices
+// before it), N / 4 butterflies; `first` takes the inputs from x0.
 template <int N>
-__device__ __forceinline__ void fft_row(const float2 (&x0)[N / 256][4], float2 (&v)[N / 64], float2 *buf,
+__device__ __forceinline__ void radix4_iter(const float2 (&x0)[b4<N>()][4], bool first, const float2 *src,
+                                            float2 *dst, int Ns, const float2 *tw, int lane, float s)
+{
+#pragma unroll
+    for (int b = 0; b < b4<N>(); ++b) {
+        const int j = lane + 64 * b;
+        if (j >= N / 4) continue;
+        float2 x[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = first ? x0[b][r] : src[j + r * (N / 4)];
+        const int jm = j % Ns;
+        const int t = jm * (N / (4 * Ns));
+#pragma unroll
+        for (int r = 1; r < 4; ++r) x[r] = cmul(x[r], tw[t * r]);
+        const float2 a = cadd(x[0], x[2]), bb = csub(x[0], x[2]), c = cadd(x[1], x[3]);
+        const float2 dd = csub(x[1], x[3]);
+        const float2 d = make_float2(-s * dd.y, s * dd.x);              // (x1 - x3) * (s i)
+        const int idx = (j / Ns) * Ns * 4 + jm;
+        dst[idx] = cadd(a, c);
+        dst[idx + Ns] = cadd(bb, d);
+        dst[idx + 2 * Ns] = csub(a, c);
+        dst[idx + 3 * Ns] = csub(bb, d);
+    }
+}
+
+// N = 384 = 4 4 4 2 3: three radix-4 iterations, a radix-2 one (Ns = 64) and
+// a final radix-3 one (Ns = 128) whose outputs j + 128 r are lane l's
+// v[b + 2 r] (j = l + 64 b) in natural order.
+__device__ __forceinline__ void fft_row384(const float2 (&x0)[b4<384>()][4], float2 (&v)[6], float2 *buf,
+                                           const float2 *tw, int lane, float s)
+{
+    constexpr int N = 384;
+    float2 *p = buf, *q = buf + N;
+    radix4_iter<N>(x0, true, nullptr, p, 1, tw, lane, s);
+    radix4_iter<N>(x0, false, p, q, 4, tw, lane, s);
+    radix4_iter<N>(x0, false, q, p, 16, tw, lane, s);
+    // radix 2, Ns = 64: N / 2 = 192 butterflies
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const int j = lane + 64 * b;
+        const int jm = j % 64;
+        const float2 a = p[j];
+        const float2 c = cmul(p[j + N / 2], tw[jm * (N / 128)]);
+        const int idx = (j / 64) * 128 + jm;
+        q[idx] = cadd(a, c);
+        q[idx + 64] = csub(a, c);
+    }
+    // radix 3, Ns = 128: N / 3 = 128 butterflies; w = exp(s 2 pi i / 3)
+    const float h = 0.866025403784438647f * s;                      // s sqrt(3) / 2
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int j = lane + 64 * b;
+        const float2 x0v = q[j];
+        const float2 x1 = cmul(q[j + 128], tw[j]);
+        const float2 x2 = cmul(q[j + 256], tw[2 * j]);
+        const float2 sum = cadd(x1, x2), dif = csub(x1, x2);
+        const float2 m = make_float2(__fsub_rn(x0v.x, __fmul_rn(0.5f, sum.x)), __fsub_rn(x0v.y, __fmul_rn(0.5f, sum.y)));
+        const float2 rot = make_float2(-__fmul_rn(h, dif.y), __fmul_rn(h, dif.x));   // i s sqrt(3)/2 (x1 - x2)
+        v[b] = cadd(x0v, sum);
+        v[b + 2] = cadd(m, rot);
+        v[b + 4] = csub(m, rot);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void fft_row(const float2 (&x0)[b4<N>()][4], float2 (&v)[N / 64], float2 *buf,
                                         const float2 *tw, int lane, float s)
 {
-    constexpr int Q = N / 64;                    // values per lane
-    constexpr int B4 = N / 4 / 64;               // radix-4 butterflies per lane
-    float2 *src = buf, *dst = buf + N;
-    int Ns = 1;
-    bool first = true;
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-        if (Ns * 4 > N) break;
-#pragma unroll
-        for (int b = 0; b < B4; ++b) {
-            const int j = lane + 64 * b;
-            float2 x[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) x[r] = first ? x0[b][r] : src[j + r * (N / 4)];
-            const int jm = j % Ns;
-            const int t = jm * (N / (4 * Ns));
-#pragma unroll
-            for (int r = 1; r < 4; ++r) x[r] = cmul(x[r], tw[(t * r) & (N - 1)]);
-            const float2 a = cadd(x[0], x[2]), bb = csub(x[0], x[2]), c = cadd(x[1], x[3]);
-            const float2 dd = csub(x[1], x[3]);
-            const float2 d = make_float2(-s * dd.y, s * dd.x);          // (x1 - x3) * (s i)
-            const int idx = (j / Ns) * Ns * 4 + jm;
-            dst[idx] = cadd(a, c);
-            dst[idx + Ns] = cadd(bb, d);
-            dst[idx + 2 * Ns] = csub(a, c);
-            dst[idx + 3 * Ns] = csub(bb, d);
-        }
-        first = false;
-        float2 *tmp = src; src = dst; dst = tmp;
-        Ns *= 4;
-    }
-    if (Ns < N) {
-        // N = 2 * 4^m: one radix-2 iteration with Ns = N / 2 (output index = j)
-#pragma unroll
-        for (int b = 0; b < N / 2 / 64; ++b) {
-            const int j = lane + 64 * b;
-            const float2 x0 = src[j];
-            const float2 x1 = cmul(src[j + N / 2], tw[j]);
-            v[b] = cadd(x0, x1);
-            v[b + N / 128] = csub(x0, x1);
-        }
+    if constexpr (N == 384) {
+        fft_row384(x0, v, buf, tw, lane, s);
     } else {
-#pragma unroll
-        for (int q = 0; q < Q; ++q) v[q] = src[lane + 64 * q];
+        constexpr int Q = N / 64;                    // values per lane
+        constexpr int B4 = N / 4 / 64;               // radix-4 butterflies per lane
+        float2 *src = buf, *dst = buf + N;
+        int Ns = 1;
+        bool first = true;
+    #pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            if (Ns * 4 > N) break;
+    #pragma unroll
+            for (int b = 0; b < B4; ++b) {
+                const int j = lane + 64 * b;
+                float2 x[4];
+    #pragma unroll
+                for (int r = 0; r < 4; ++r) x[r] = first ? x0[b][r] : src[j + r * (N / 4)];
+                const int jm = j % Ns;
+                const int t = jm * (N / (4 * Ns));
+    #pragma unroll
+                for (int r = 1; r < 4; ++r) x[r] = cmul(x[r], tw[(t * r) & (N - 1)]);
+                const float2 a = cadd(x[0], x[2]), bb = csub(x[0], x[2]), c = cadd(x[1], x[3]);
+                const float2 dd = csub(x[1], x[3]);
+                const float2 d = make_float2(-s * dd.y, s * dd.x);          // (x1 - x3) * (s i)
+                const int idx = (j / Ns) * Ns * 4 + jm;
+                dst[idx] = cadd(a, c);
+                dst[idx + Ns] = cadd(bb, d);
+                dst[idx + 2 * Ns] = csub(a, c);
+                dst[idx + 3 * Ns] = csub(bb, d);
+            }
+            first = false;
+            float2 *tmp = src; src = dst; dst = tmp;
+            Ns *= 4;
+        }
+        if (Ns < N) {
+            // N = 2 * 4^m: one radix-2 iteration with Ns = N / 2 (output index = j)
+    #pragma unroll
+            for (int b = 0; b < N / 2 / 64; ++b) {
+                const int j = lane + 64 * b;
+                const float2 x0 = src[j];
+                const float2 x1 = cmul(src[j + N / 2], tw[j]);
+                v[b] = cadd(x0, x1);
+                v[b + N / 128] = csub(x0, x1);
+            }
+        } else {
+    #pragma unroll
+            for (int q = 0; q < Q; ++q) v[q] = src[lane + 64 * q];
+        }
     }
 }
 
@@ -147,13 +224,13 @@ __global__ void __launch_bounds__(WGT) k_fft_pass(const float2 *__restrict__ in,
     const int64_t tt = T % d.nt, b = (T / d.nt) % d.nb, a = T / (d.nt * d.nb);
     const float2 *base = in + a * d.inA + b * d.inB + tt * TR * d.rowStride;
     // rows w, w + NW, .. of the tile, each wave's next row loaded ahead
-    float2 xa[N / 256][4], xb[N / 256][4];
+    float2 xa[b4<N>()][4], xb[b4<N>()][4];
     load_row<N>(base + (int64_t)w * d.rowStride, xa, lane);
 #pragma unroll
     for (int i = 0; i < TR / NW; ++i) {
         const int t = w + NW * i;
-        float2 (&cur)[N / 256][4] = (i & 1) ? xb : xa;
-        float2 (&nxt)[N / 256][4] = (i & 1) ? xa : xb;
+        float2 (&cur)[b4<N>()][4] = (i & 1) ? xb : xa;
+        float2 (&nxt)[b4<N>()][4] = (i & 1) ? xa : xb;
         if (i + 1 < TR / NW) load_row<N>(base + (int64_t)(t + NW) * d.rowStride, nxt, lane);
         float2 v[N / 64];
         fft_row<N>(cur, v, fbuf[w], tw, lane, s);
@@ -193,15 +270,15 @@ __global__ void __launch_bounds__(WGT) k_fft_sos(const float2 *__restrict__ in, 
     const int64_t tt = T % nt, a = T / nt;
     // rows (c, w + NW i) in coil order, each wave's next row loaded ahead
     const float2 *tbase = in + a * inA + tt * TS * rowStride;
-    float2 xa[N / 256][4], xb[N / 256][4];
+    float2 xa[b4<N>()][4], xb[b4<N>()][4];
     load_row<N>(tbase + (int64_t)w * rowStride, xa, lane);
 #pragma unroll 1
     for (int c = 0; c < C; ++c) {
 #pragma unroll
         for (int i = 0; i < TS / NW; ++i) {
             const int t = w + NW * i;
-            float2 (&cur)[N / 256][4] = (i & 1) ? xb : xa;
-            float2 (&nxt)[N / 256][4] = (i & 1) ? xa : xb;
+            float2 (&cur)[b4<N>()][4] = (i & 1) ? xb : xa;
+            float2 (&nxt)[b4<N>()][4] = (i & 1) ? xa : xb;
             // next row: (c, t + NW), or (c + 1, w) after the last of this coil
             if (i + 1 < TS / NW)
                 load_row<N>(tbase + c * inC + (int64_t)(t + NW) * rowStride, nxt, lane);
@@ -301,7 +378,7 @@ fr_status fr_fft_workspace(fr_fft *f, size_t bytes)
 bool fr_fft_sos_fused(const fr_fft *f, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz)
 {
     (void)rcz;
-    return (f->N == 256 || f->N == 512) && g->N == f->N && rcx % 16 == 0 && rcx >= 16 && rcy >= 1 &&
+    return (f->N == 256 || f->N == 384 || f->N == 512) && g->N == f->N && rcx % 16 == 0 && rcx >= 16 && rcy >= 1 &&
            (g->dim == 2 || rcz >= 1) && (int64_t)g->C * std::max(g->frames, 1) == f->batch;
 }
 
@@ -351,5 +428,6 @@ extern "C" fr_status fr_fft_sos(fr_fft *f, fr_grids *g, unsigned rcx, unsigned r
     fr_status e = fr_fft_tw(f);
     if (e) return e;
     if (f->N == 256) return run_passes<256>(f, g, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev);
+    if (f->N == 384) return run_passes<384>(f, g, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev);
     return run_passes<512>(f, g, (int)rcx, (int)rcy, (int)rcz, apply_shift, deapod_dev, out_dev);
 }

@@ -192,7 +192,7 @@ fr_status fr_fft_execute(fr_fft *f, fr_grids *g);
 /* FFT + fftShift + crop + sum-of-squares in one call: out_dev receives what
  * fr_fft_execute, then fr_sos(..., apply_shift, deapod_dev, out_dev) would
  * give (FFT.cpp:82-84, ImageFilter.cpp:40-53,159-209, Main.cpp:175-181), to
- * FP32 rounding.  For N = 256 or 512 and rcx a multiple of 16 it runs
+ * FP32 rounding.  For N = 256, 384 or 512 and rcx a multiple of 16 it runs
  * cropped transforms that keep only the rows and columns reaching the crop
  * (the grids are left untransformed); otherwise it runs exactly those two
  * calls (the grids are left transformed).  No counterpart in the reference. */

@@ -208,10 +208,14 @@ def _rand_grids(ctx, dim, N, C, seed, frames=0):
     (2, 512, 3, (256, 256, 1), True, False, 0),
     (2, 512, 2, (192, 160, 1), False, True, 0),
     (2, 256, 3, (128, 128, 1), True, False, 5),
+    (3, 384, 2, (256, 256, 256), True, True, 0),    # N = 384: radices 4, 4, 4, 2, 3
+    (3, 384, 1, (192, 160, 176), False, False, 0),
+    (2, 384, 3, (192, 192, 1), True, False, 4),
+    (2, 384, 2, (256, 240, 1), False, True, 0),
     (3, 128, 2, (64, 64, 64), True, False, 0),      # not a cropped-path size: the FFT + SOS fallback
 ])
 def test_fft_sos_matches_fft_then_sos(ctx, dim, N, C, rc, shift, deapod, frames):
-    """fr_fft_sos (cropped transforms for N = 256, 512) equals fr_fft_execute
+    """fr_fft_sos (cropped transforms for N = 256, 384, 512) equals fr_fft_execute
     followed by fr_sos on the same grids to FP32 rounding."""
     g, data = _rand_grids(ctx, dim, N, C, 31 + N + C, frames)
     batch = C * max(frames, 1)
@@ -247,4 +251,18 @@ def test_fft_sos_against_oracle(ctx, orc):
     ref = orc.sos(3, N, img, rc)
     e = rel_l2(got, ref)
     print(f"[fft_sos vs FP64 oracle] rel-L2 {e:.2e}", flush=True)
+    assert e <= TOL
+
+
+def test_fft_sos_384_against_oracle(ctx, orc):
+    """The mixed-radix (4, 4, 4, 2, 3) N = 384 transform against the FP64
+    oracle FFT, fftshift and SOS, 2D with three coils."""
+    N, C, rc = 384, 3, (256, 256, 1)
+    g, data = _rand_grids(ctx, 2, N, C, 78)
+    f = hip.FFT(ctx, 2, N, C, +1)
+    got = hip.fft_sos(f, g, rc, True).to_host(np.float32)
+    img = orc.fftshift(2, N, orc.fft_c2c(2, N, data, +1))
+    ref = orc.sos(2, N, img, rc)
+    e = rel_l2(got, ref)
+    print(f"[fft_sos N=384 vs FP64 oracle] rel-L2 {e:.2e}", flush=True)
     assert e <= TOL
