@@ -3,6 +3,12 @@
 This is plumbing for Python callers (tests, bench): every compute call goes
 through the native library.  There is no Python or CPU fallback: if the
 library is missing, loading it raises.
+
+A process that also uses PyTorch's GPU side must import torch before loading
+this library: torch ships its own HIP and HSA runtime libraries under the same
+sonames as ROCm's, the copy loaded first serves both, and torch finds no
+device when ROCm's copy came first (bench.py and fastrecon_amd.ldcf.evaluate
+follow this order).
 """
 import ctypes as C
 import os

@@ -197,7 +197,9 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     upload): end-to-end milliseconds of each recon from the plan (iterative:
     Pipe-Menon + grid + FFT + SOS, on the recon's plan and on a second plan
     shaped for single-channel gridding; learned: features + MLP + grid + FFT +
-    SOS), image NRMSE of learned (and ramp) against iterative, PSF sidelobes."""
+    SOS), image NRMSE of learned (and ramp) against iterative, PSF sidelobes.
+    Times with torch's HIP events, so torch must have been imported before
+    the library was loaded (the binding's module docstring says why)."""
     import torch  # device timing through HIP events on the context stream
 
     tn, _ = hip.normalize_traj(traj, N)
