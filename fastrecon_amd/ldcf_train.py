@@ -1,11 +1,11 @@
 """Train the learned-DCF network on the device and write the checkpoint.
 
-    python -m fastrecon_amd.ldcf_train [--out PATH] [--count 200] [--per-traj 50000] [--steps 4000]
+    python -m fastrecon_amd.ldcf_train [--out PATH] [--count 200] [--per-traj 50000] [--steps 12000]
 
 Training set: `count` seeded variable-density 3D radial and cones
 trajectories for a 128^3 image (256^3 grid), with 10-iteration Pipe-Menon
 DCFs from this package's gridding/degridding kernels as targets
-(fastrecon_amd.ldcf.build_dataset).  The network trains with the package's
+(fastrecon_amd.ldcf.build_dataset; rows drawn by ldcf.training_rows).  The network trains with the package's
 own BF16 forward/backward kernels and Adam (cosine learning-rate decay)."""
 import argparse
 import json
@@ -26,7 +26,7 @@ def main():
     ap.add_argument("--count", type=int, default=200)
     ap.add_argument("--per-traj", type=int, default=50000)
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=7)
@@ -48,7 +48,8 @@ def main():
     meta = dict(count=a.count, per_traj=a.per_traj, scale=a.scale, steps=a.steps, batch=a.batch, lr=a.lr,
                 seed=a.seed, rows=int(X.shape[0]), dataset_s=round(t_data, 1), train_s=round(t_train, 1),
                 losses=[(int(s), float(l)) for s, l in losses], grid=ldcf.N_GRID, image=ldcf.IMAGE,
-                targets="10-iteration Pipe-Menon DCF", features="|u|, azimuth, polar, log1p(n_1,2,4)/8")
+                targets="10-iteration Pipe-Menon DCF", features="|u|, azimuth, polar, log1p(n_1,2,4)/8",
+                rows="1/2 radius-stratified, 1/4 log-radius-stratified, 1/4 drawn ~ w^2")
     ldcf.save(net, a.out, meta)
     log(f"[ldcf] trained in {t_train:.1f} s, final loss {losses[-1][1]:.5f}; wrote {a.out}")
     log(json.dumps(meta))
