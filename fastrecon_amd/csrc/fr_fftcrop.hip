@@ -16,7 +16,7 @@
 // optional deapodisation divide: the arithmetic of k_sos (fr_image.hip).
 //
 // Row transform: N-point Stockham radix-4 (N = 4^m, or 2 * 4^m with a final
-// radix-2 step; N = 384 takes radices 4, 4, 4, 2, 3), one wave per row, twiddles exp(+2 pi i k / N) from an FP64
+// radix-2 step; N = 384 takes radices 6, 8, 8), one wave per row, twiddles exp(+2 pi i k / N) from an FP64
 // table rounded to FP32.  The result differs from rocFFT's in rounding
 // (relative ~1e-7), well inside the 1e-5 image bar.
 #include "fr_internal.h"
@@ -46,9 +46,18 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(__fadd_rn(a.x, b.x), __fadd_rn(a.y, b.y)); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y)); }
 
-// radix-4 butterflies per lane in one iteration (N / 4 of them; for N = 384
-// the second set covers lanes 0 .. 31 only)
+// Input slots per lane: b4 sets of 4 (the first radix-4 iteration's inputs;
+// N = 384 keeps its 6 first-stage inputs in the first 6 slots)
 template <int N> constexpr int b4() { return (N / 4 + 63) / 64; }
+// Outputs per lane after a row transform, and which output k lane l holds in
+// v[q]: k = l + 64 q, or for N = 384 (last stage radix 8 over 48 lanes)
+// k = l + 48 q on lanes 0 .. 47.
+template <int N> constexpr int nv() { return N == 384 ? 8 : N / 64; }
+template <int N> __device__ __forceinline__ int out_k(int lane, int q)
+{
+    if constexpr (N == 384) return lane < 48 ? lane + 48 * q : -1;
+    else return lane + 64 * q;
+}
 
 // The first radix-4 iteration's inputs of the row at `in` (unit stride):
 // lane l, butterfly b takes elements j + r N/4, j = l + 64 b.  Loaded one row
@@ -56,11 +65,18 @@ template <int N> constexpr int b4() { return (N / 4 + 63) / 64; }
 template <int N>
 __device__ __forceinline__ void load_row(const float2 *__restrict__ in, float2 (&x0)[b4<N>()][4], int lane)
 {
+    if constexpr (N == 384) {
+        // radix-6 first stage: lane l takes elements l + 64 r, r = 0 .. 5
 #pragma unroll
-    for (int b = 0; b < b4<N>(); ++b) {
-        const int j = lane + 64 * b;
+        for (int r = 0; r < 6; ++r) x0[r >> 2][r & 3] = in[lane + 64 * r];
+        x0[1][2] = x0[1][3] = make_float2(0.f, 0.f);
+    } else {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) x0[b][r] = j < N / 4 ? in[j + r * (N / 4)] : make_float2(0.f, 0.f);
+        for (int b = 0; b < b4<N>(); ++b) {
+            const int j = lane + 64 * b;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x0[b][r] = in[j + r * (N / 4)];
+        }
     }
 }
 
@@ -68,75 +84,115 @@ __device__ __forceinline__ void load_row(const float2 *__restrict__ in, float2 (
 // in x0 (load_row).  After the call, lane l holds output k = l + 64 q in v[q]
 // (q = 0 .. N/64 - 1).  buf: 2N float2 of this wave's LDS; tw: N twiddles
 // exp(s 2 pi i k / N).
-// One Stockham radix-4 iteration over stride Ns (the product of the radices
-// before it), N / 4 butterflies; `first` takes the inputs from x0.
-template <int N>
-__device__ __forceinline__ void radix4_iter(const float2 (&x0)[b4<N>()][4], bool first, const float2 *src,
-                                            float2 *dst, int Ns, const float2 *tw, int lane, float s)
+__device__ __forceinline__ float2 mul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }  // a (s i)
+
+// 3-point DFT with w = exp(s 2 pi i / 3), h = s sqrt(3) / 2
+__device__ __forceinline__ void dft3(float2 b0, float2 b1, float2 b2, float h, float2 &X0, float2 &X1, float2 &X2)
 {
-#pragma unroll
-    for (int b = 0; b < b4<N>(); ++b) {
-        const int j = lane + 64 * b;
-        if (j >= N / 4) continue;
-        float2 x[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) x[r] = first ? x0[b][r] : src[j + r * (N / 4)];
-        const int jm = j % Ns;
-        const int t = jm * (N / (4 * Ns));
-#pragma unroll
-        for (int r = 1; r < 4; ++r) x[r] = cmul(x[r], tw[t * r]);
-        const float2 a = cadd(x[0], x[2]), bb = csub(x[0], x[2]), c = cadd(x[1], x[3]);
-        const float2 dd = csub(x[1], x[3]);
-        const float2 d = make_float2(-s * dd.y, s * dd.x);              // (x1 - x3) * (s i)
-        const int idx = (j / Ns) * Ns * 4 + jm;
-        dst[idx] = cadd(a, c);
-        dst[idx + Ns] = cadd(bb, d);
-        dst[idx + 2 * Ns] = csub(a, c);
-        dst[idx + 3 * Ns] = csub(bb, d);
-    }
+    const float2 sum = cadd(b1, b2), dif = csub(b1, b2);
+    const float2 m = make_float2(__fsub_rn(b0.x, __fmul_rn(0.5f, sum.x)), __fsub_rn(b0.y, __fmul_rn(0.5f, sum.y)));
+    const float2 rot = make_float2(-__fmul_rn(h, dif.y), __fmul_rn(h, dif.x));      // i h (b1 - b2)
+    X0 = cadd(b0, sum);
+    X1 = cadd(m, rot);
+    X2 = csub(m, rot);
 }
 
-// N = 384 = 4 4 4 2 3: three radix-4 iterations, a radix-2 one (Ns = 64) and
-// a final radix-3 one (Ns = 128) whose outputs j + 128 r are lane l's
-// v[b + 2 r] (j = l + 64 b) in natural order.
-__device__ __forceinline__ void fft_row384(const float2 (&x0)[b4<384>()][4], float2 (&v)[6], float2 *buf,
+// 4-point DFT with w = exp(s 2 pi i / 4) = s i
+__device__ __forceinline__ void dft4(float2 a0, float2 a1, float2 a2, float2 a3, float s, float2 (&X)[4])
+{
+    const float2 a = cadd(a0, a2), bb = csub(a0, a2), c = cadd(a1, a3), d = mul_si(csub(a1, a3), s);
+    X[0] = cadd(a, c);
+    X[1] = cadd(bb, d);
+    X[2] = csub(a, c);
+    X[3] = csub(bb, d);
+}
+
+// 6-point DFT, w6 = exp(s 2 pi i / 6): n = n2 + 3 n1, k = k1 + 2 k2; size-2
+// DFTs over n1, twiddles w6^(n2 k1), size-3 DFTs over n2
+__device__ __forceinline__ void dft6(float2 (&x)[6], float s)
+{
+    const float h = 0.866025403784438647f * s;                       // s sqrt(3) / 2
+    float2 e[3], o[3];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+        e[n2] = cadd(x[n2], x[n2 + 3]);
+        o[n2] = csub(x[n2], x[n2 + 3]);
+    }
+    // o1 *= w6 = (1/2, h); o2 *= w6^2 = (-1/2, h)
+    const float2 o1 = make_float2(__fsub_rn(__fmul_rn(0.5f, o[1].x), __fmul_rn(h, o[1].y)),
+                                  __fadd_rn(__fmul_rn(0.5f, o[1].y), __fmul_rn(h, o[1].x)));
+    const float2 o2 = make_float2(__fsub_rn(__fmul_rn(-0.5f, o[2].x), __fmul_rn(h, o[2].y)),
+                                  __fadd_rn(__fmul_rn(-0.5f, o[2].y), __fmul_rn(h, o[2].x)));
+    dft3(e[0], e[1], e[2], h, x[0], x[2], x[4]);
+    dft3(o[0], o1, o2, h, x[1], x[3], x[5]);
+}
+
+// 8-point DFT, w8 = exp(s 2 pi i / 8): even and odd 4-point DFTs, then
+// X[k] = E[k] + w8^k O[k], X[k + 4] = E[k] - w8^k O[k]
+__device__ __forceinline__ void dft8(float2 (&x)[8], float s)
+{
+    const float c = 0.707106781186547524f;                            // sqrt(1/2)
+    float2 E[4], O[4];
+    dft4(x[0], x[2], x[4], x[6], s, E);
+    dft4(x[1], x[3], x[5], x[7], s, O);
+    const float sc = s * c;
+    // w8 = (c, s c), w8^2 = s i, w8^3 = (-c, s c)
+    const float2 t1 = make_float2(__fsub_rn(__fmul_rn(c, O[1].x), __fmul_rn(sc, O[1].y)),
+                                  __fadd_rn(__fmul_rn(c, O[1].y), __fmul_rn(sc, O[1].x)));
+    const float2 t2 = mul_si(O[2], s);
+    const float2 t3 = make_float2(__fsub_rn(__fmul_rn(-c, O[3].x), __fmul_rn(sc, O[3].y)),
+                                  __fadd_rn(__fmul_rn(-c, O[3].y), __fmul_rn(sc, O[3].x)));
+    x[0] = cadd(E[0], O[0]);
+    x[4] = csub(E[0], O[0]);
+    x[1] = cadd(E[1], t1);
+    x[5] = csub(E[1], t1);
+    x[2] = cadd(E[2], t2);
+    x[6] = csub(E[2], t2);
+    x[3] = cadd(E[3], t3);
+    x[7] = csub(E[3], t3);
+}
+
+// N = 384 = 6 8 8 (Stockham): a radix-6 stage (Ns = 1, 64 butterflies, one
+// per lane, inputs from x0), a radix-8 stage (Ns = 6, 48 butterflies) and a
+// final radix-8 stage (Ns = 48, 48 butterflies) whose outputs j + 48 r are
+// lane j's v[r] (lanes 48 .. 63 hold none).  Twiddles of a radix-R stage:
+// exp(s 2 pi i r (j mod Ns) / (R Ns)) = tw[r (j mod Ns) N / (R Ns)].
+__device__ __forceinline__ void fft_row384(const float2 (&x0)[b4<384>()][4], float2 (&v)[8], float2 *buf,
                                            const float2 *tw, int lane, float s)
 {
-    constexpr int N = 384;
-    float2 *p = buf, *q = buf + N;
-    radix4_iter<N>(x0, true, nullptr, p, 1, tw, lane, s);
-    radix4_iter<N>(x0, false, p, q, 4, tw, lane, s);
-    radix4_iter<N>(x0, false, q, p, 16, tw, lane, s);
-    // radix 2, Ns = 64: N / 2 = 192 butterflies
+    float2 *p = buf, *q = buf + 384;
+    {
+        float2 x[6];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-        const int j = lane + 64 * b;
-        const int jm = j % 64;
-        const float2 a = p[j];
-        const float2 c = cmul(p[j + N / 2], tw[jm * (N / 128)]);
-        const int idx = (j / 64) * 128 + jm;
-        q[idx] = cadd(a, c);
-        q[idx + 64] = csub(a, c);
+        for (int r = 0; r < 6; ++r) x[r] = x0[r >> 2][r & 3];
+        dft6(x, s);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) p[6 * lane + r] = x[r];
     }
-    // radix 3, Ns = 128: N / 3 = 128 butterflies; w = exp(s 2 pi i / 3)
-    const float h = 0.866025403784438647f * s;                      // s sqrt(3) / 2
+    if (lane < 48) {
+        const int jm = lane % 6;
+        float2 x[8];
+        x[0] = p[lane];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int j = lane + 64 * b;
-        const float2 x0v = q[j];
-        const float2 x1 = cmul(q[j + 128], tw[j]);
-        const float2 x2 = cmul(q[j + 256], tw[2 * j]);
-        const float2 sum = cadd(x1, x2), dif = csub(x1, x2);
-        const float2 m = make_float2(__fsub_rn(x0v.x, __fmul_rn(0.5f, sum.x)), __fsub_rn(x0v.y, __fmul_rn(0.5f, sum.y)));
-        const float2 rot = make_float2(-__fmul_rn(h, dif.y), __fmul_rn(h, dif.x));   // i s sqrt(3)/2 (x1 - x2)
-        v[b] = cadd(x0v, sum);
-        v[b + 2] = cadd(m, rot);
-        v[b + 4] = csub(m, rot);
+        for (int r = 1; r < 8; ++r) x[r] = cmul(p[lane + 48 * r], tw[8 * r * jm]);
+        dft8(x, s);
+        const int idx = (lane / 6) * 48 + jm;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) q[idx + 6 * r] = x[r];
+    }
+    if (lane < 48) {
+        v[0] = q[lane];
+#pragma unroll
+        for (int r = 1; r < 8; ++r) v[r] = cmul(q[lane + 48 * r], tw[r * lane]);
+        dft8(v, s);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = make_float2(0.f, 0.f);
     }
 }
 
 template <int N>
-__device__ __forceinline__ void fft_row(const float2 (&x0)[b4<N>()][4], float2 (&v)[N / 64], float2 *buf,
+__device__ __forceinline__ void fft_row(const float2 (&x0)[b4<N>()][4], float2 (&v)[nv<N>()], float2 *buf,
                                         const float2 *tw, int lane, float s)
 {
     if constexpr (N == 384) {
@@ -232,11 +288,12 @@ __global__ void __launch_bounds__(WGT) k_fft_pass(const float2 *__restrict__ in,
         float2 (&cur)[b4<N>()][4] = (i & 1) ? xb : xa;
         float2 (&nxt)[b4<N>()][4] = (i & 1) ? xa : xb;
         if (i + 1 < TR / NW) load_row<N>(base + (int64_t)(t + NW) * d.rowStride, nxt, lane);
-        float2 v[N / 64];
+        float2 v[nv<N>()];
         fft_row<N>(cur, v, fbuf[w], tw, lane, s);
 #pragma unroll
-        for (int q = 0; q < N / 64; ++q) {
-            const int kp = crop_pos(lane + 64 * q, N, rc, shift);
+        for (int q = 0; q < nv<N>(); ++q) {
+            const int k = out_k<N>(lane, q);
+            const int kp = k < 0 ? -1 : crop_pos(k, N, rc, shift);
             if (kp >= 0) otile[kp * TR + t] = v[q];
         }
     }
@@ -284,11 +341,12 @@ __global__ void __launch_bounds__(WGT) k_fft_sos(const float2 *__restrict__ in, 
                 load_row<N>(tbase + c * inC + (int64_t)(t + NW) * rowStride, nxt, lane);
             else if (c + 1 < C)
                 load_row<N>(tbase + (c + 1) * inC + (int64_t)w * rowStride, nxt, lane);
-            float2 v[N / 64];
+            float2 v[nv<N>()];
             fft_row<N>(cur, v, fbuf[w], tw, lane, s);
 #pragma unroll
-            for (int q = 0; q < N / 64; ++q) {
-                const int kp = crop_pos(lane + 64 * q, N, rc, shift);
+            for (int q = 0; q < nv<N>(); ++q) {
+                const int k = out_k<N>(lane, q);
+                const int kp = k < 0 ? -1 : crop_pos(k, N, rc, shift);
                 if (kp >= 0) {
                     float &x = acc[kp * TS + t];
                     x = __fadd_rn(x, __fadd_rn(__fmul_rn(v[q].x, v[q].x), __fmul_rn(v[q].y, v[q].y)));
