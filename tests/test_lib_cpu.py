@@ -54,3 +54,26 @@ def test_invalid_arguments_rejected_on_host():
     assert L.fr_kb_lut(-1.0, 2.0, 4, out.ctypes.data_as(ctypes.c_void_p), None) == 1
     assert L.fr_traj_normalize(None, 10, 2, 2, 64, None) == 1
     assert L.fr_ctx_create(0, None) == 1
+
+
+def test_python_sources_compile():
+    """Every Python file of the package, the trainer, bench and entry point
+    byte-compiles (catches errors in modules no CPU test imports)."""
+    files = [os.path.join(ROOT, f) for f in ("bench.py", "__graft_entry__.py")]
+    for d in ("fastrecon_amd", "oracle", "tests"):
+        for dp, _, fs in os.walk(os.path.join(ROOT, d)):
+            files += [os.path.join(dp, f) for f in fs if f.endswith(".py")]
+    for f in files:
+        with open(f) as fh:
+            compile(fh.read(), f, "exec")
+
+
+def test_checkpoint_metadata():
+    """The shipped learned-DCF checkpoint loads without pickle and carries
+    the row count and training recipe the bench reports."""
+    import json
+    z = np.load(os.path.join(ROOT, "fastrecon_amd", "ldcf_kooshball.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    assert isinstance(meta["rows"], int) and meta["rows"] > 0
+    assert meta["steps"] > 0 and meta["losses"][-1][1] > 0
+    assert z["params"].dtype == np.float32 and z["params"].size == 6 * 64 + 64 + 64 * 64 + 64 + 64 + 1
