@@ -8,7 +8,7 @@ that this package's own gridding and degridding kernels produce
 (fr_dcf_pipe_menon), on a seeded set of variable-density 3D radial and cones
 trajectories for a 128^3 image (256^3 grid).  Training runs on the device
 with the package's own BF16 forward/backward kernels and Adam
-(fr_mlp_train_step).  The iterative DCF stays the fallback and the judge:
+(fr_mlp_train_step).  The iterative DCF stays the fallback and the reference:
 image NRMSE against the iterative-DCF image and the PSF sidelobe level.
 """
 import json
@@ -26,7 +26,7 @@ N_GRID = syn.grid_size(IMAGE, OG)
 CHECKPOINT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ldcf_kooshball.npz")
 
 
-# ----------------------------------------------------------------- judges
+# ------------------------------------------------------- quality measures
 def nrmse(a, b):
     """||a - b|| / ||b|| (no scale fit)."""
     a = np.asarray(a, np.float64).ravel()

@@ -30,7 +30,7 @@ def main():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--eval", action="store_true", help="judge the result on the kooshball phantom")
+    ap.add_argument("--eval", action="store_true", help="evaluate the result on the kooshball phantom")
     a = ap.parse_args()
 
     def log(msg):
@@ -54,7 +54,7 @@ def main():
     log(f"[ldcf] trained in {t_train:.1f} s, final loss {losses[-1][1]:.5f}; wrote {a.out}")
     log(json.dumps(meta))
     if a.eval:
-        # judge on the 32-coil kooshball geometry (51,200 spokes x 256, 256^3 grid)
+        # evaluate on the 32-coil kooshball geometry (51,200 spokes x 256, 256^3 grid)
         from fastrecon_amd import synthetic as syn
         traj, ramp = syn.kooshball3d(51200, 256)
         r = ldcf.evaluate_phantom(ctx, net, traj, ldcf.N_GRID, (ldcf.IMAGE,) * 3, ldcf.IMAGE, ramp=ramp)

@@ -3,7 +3,7 @@
 Bar: in exact-order mode, bit-identical to the oracle's FP32 restatement of
 the reference CPU loop (GridLut.cpp:80-147); in the default mode (heavy units
 split by sample range and combined in a fixed order) relative L2 <= 1e-5, the
-north-star FP32 tolerance, and run-to-run bitwise determinism."""
+target FP32 tolerance, and run-to-run bitwise determinism."""
 import numpy as np
 import pytest
 

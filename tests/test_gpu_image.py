@@ -2,7 +2,7 @@
 C ABI: batched rocFFT (sign +1, unnormalised), fftshift, crop + SOS, the
 deapodisation map, degridding and Pipe-Menon, against the oracle.
 
-Bar: relative L2 <= 1e-5 (north-star FP32 tolerance) for every image; the
+Bar: relative L2 <= 1e-5 (the target FP32 tolerance) for every image; the
 oracle's FFT is FP64, so these numbers include the GPU's FP32 FFT rounding."""
 import numpy as np
 import pytest

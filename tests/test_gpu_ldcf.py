@@ -2,7 +2,7 @@
 against the oracle's brute-force definition, the BF16 MLP forward and its
 gradient against the FP64 oracle (same BF16 rounding points), determinism,
 training on the device, and what the learned DCF gains over the analytic
-ramp DCF in a reconstruction judged against the Pipe-Menon (iterative) DCF.
+ramp DCF in a reconstruction measured against the Pipe-Menon (iterative) DCF.
 
 Bars: neighbour counts exact; angles 1e-5; forward |dy| <= 1e-2 (a BF16
 rounding flip of one hidden unit moves y by ~1e-3) with median <= 1e-5;
