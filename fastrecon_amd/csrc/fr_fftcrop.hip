@@ -36,7 +36,7 @@ constexpr int TS = 8;           // rows (output columns) per workgroup tile of t
 #endif
 template <int N> constexpr int waves() { return N >= 512 ? 4 : FR_FFT_WAVES; }
 static_assert(TR % (2 * FR_FFT_WAVES) == 0 && TS % (2 * FR_FFT_WAVES) == 0 && 16 % TS == 0,
-              "an even number of rows per wave (the prefetch alternates two buffers)");
+              "an even number of rows per wave (the passes load two rows ahead, the last pass alternates two buffers)");
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -279,15 +279,16 @@ __global__ void __launch_bounds__(WGT) k_fft_pass(const float2 *__restrict__ in,
     const int64_t T = blockIdx.x;
     const int64_t tt = T % d.nt, b = (T / d.nt) % d.nb, a = T / (d.nt * d.nb);
     const float2 *base = in + a * d.inA + b * d.inB + tt * TR * d.rowStride;
-    // rows w, w + NW, .. of the tile, each wave's next row loaded ahead
-    float2 xa[b4<N>()][4], xb[b4<N>()][4];
+    // rows w, w + NW, .. of the tile, each wave's rows loaded two ahead
+    float2 xa[b4<N>()][4], xb[b4<N>()][4], xc[b4<N>()][4];
     load_row<N>(base + (int64_t)w * d.rowStride, xa, lane);
+    load_row<N>(base + (int64_t)(w + NW) * d.rowStride, xb, lane);
 #pragma unroll
     for (int i = 0; i < TR / NW; ++i) {
         const int t = w + NW * i;
-        float2 (&cur)[b4<N>()][4] = (i & 1) ? xb : xa;
-        float2 (&nxt)[b4<N>()][4] = (i & 1) ? xa : xb;
-        if (i + 1 < TR / NW) load_row<N>(base + (int64_t)(t + NW) * d.rowStride, nxt, lane);
+        float2 (&cur)[b4<N>()][4] = i % 3 == 0 ? xa : (i % 3 == 1 ? xb : xc);
+        float2 (&nxt)[b4<N>()][4] = (i + 2) % 3 == 0 ? xa : ((i + 2) % 3 == 1 ? xb : xc);
+        if (i + 2 < TR / NW) load_row<N>(base + (int64_t)(t + 2 * NW) * d.rowStride, nxt, lane);
         float2 v[nv<N>()];
         fft_row<N>(cur, v, fbuf[w], tw, lane, s);
 #pragma unroll
