@@ -1,6 +1,6 @@
 """Train the learned-DCF network on the device and write the checkpoint.
 
-    python -m fastrecon_amd.ldcf_train [--out PATH] [--count 200] [--per-traj 50000] [--steps 24000] [--lr 5e-3]
+    python -m fastrecon_amd.ldcf_train [--out PATH] [--count 200] [--per-traj 50000] [--steps 48000] [--lr 5e-3]
 
 Training set: `count` seeded variable-density 3D radial and cones
 trajectories for a 128^3 image (256^3 grid), with 10-iteration Pipe-Menon
@@ -26,7 +26,7 @@ def main():
     ap.add_argument("--count", type=int, default=200)
     ap.add_argument("--per-traj", type=int, default=50000)
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--steps", type=int, default=24000)
+    ap.add_argument("--steps", type=int, default=48000)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--seed", type=int, default=7)
