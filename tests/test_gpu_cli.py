@@ -25,6 +25,7 @@ def rel_l2(a, b):
 CASES = [
     # dim, spokes, readout, coils, image, pils
     (2, 402, 512, 8, 128, False),
+    (2, 302, 384, 4, 192, False),      # 384^2 grid: the radix-6,8,8 cropped transforms behind ImageFilter::SOS
     (2, 201, 256, 4, 64, True),
     (3, 800, 64, 4, 32, False),
     (3, 800, 64, 3, 32, True),
