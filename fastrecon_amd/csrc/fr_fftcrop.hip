@@ -81,8 +81,8 @@ __device__ __forceinline__ void load_row(const float2 *__restrict__ in, float2 (
 }
 
 // Natural-order N-point transform of a row whose first-iteration inputs are
-// in x0 (load_row).  After the call, lane l holds output k = l + 64 q in v[q]
-// (q = 0 .. N/64 - 1).  buf: 2N float2 of this wave's LDS; tw: N twiddles
+// in x0 (load_row).  After the call, lane l holds output k = out_k<N>(l, q)
+// in v[q] (q = 0 .. nv<N>() - 1).  buf: 2N float2 of this wave's LDS; tw: N twiddles
 // exp(s 2 pi i k / N).
 __device__ __forceinline__ float2 mul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }  // a (s i)
 
