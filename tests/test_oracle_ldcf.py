@@ -57,3 +57,26 @@ def test_angular_features_closed_forms():
     np.testing.assert_allclose(f[:, 1], [0.0, 0.5, 0.0, -0.5, 1.0], atol=1e-7)
     np.testing.assert_allclose(f[:, 2], [0.5, 0.5, 0.0, 0.5, 0.75], atol=1e-7)
     assert np.all(f[:, 3:] == 0)           # isolated points: no neighbours
+
+
+def test_training_rows_strata():
+    """ldcf.training_rows: seeded, the requested count, half of the rows
+    spread evenly over the radius, and the w^2 stratum drawn towards the
+    heavy samples (no GPU)."""
+    from fastrecon_amd import ldcf
+    rng = np.random.default_rng(3)
+    M = 20000
+    r = rng.uniform(0.0, 128.0, M)
+    feat = np.zeros((M, 8), np.float32)
+    feat[:, 0] = np.log1p(r) / 5.0
+    w = (r / 128.0) ** 2 + 1e-3
+    a = ldcf.training_rows(feat, w, 4000, 7, 5)
+    b = ldcf.training_rows(feat, w, 4000, 7, 5)
+    assert a.shape == (4000,) and np.array_equal(a, b)
+    assert a.min() >= 0 and a.max() < M
+    # radius stratum: the first half covers the 32 radius bins about evenly
+    rr = np.expm1(5.0 * feat[a[:2000], 0].astype(np.float64))
+    counts = np.histogram(rr, bins=32, range=(0.0, rr.max() + 1e-9))[0]
+    assert counts.min() >= 0.5 * counts.mean()
+    # w^2 stratum (last quarter): mean w^2 well above the population's
+    assert np.mean(w[a[3000:]] ** 2) > 1.5 * np.mean(w ** 2)
