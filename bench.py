@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--unit", type=int, nargs=3, default=None, metavar=("X", "Y", "Z"),
                     help="base work-unit box (default: the plan's choice)")
     ap.add_argument("--split", type=int, default=0, help="slice/split threshold in taps (0: automatic)")
+    ap.add_argument("--ipw", type=int, default=0, help="accumulate 6: light items per wave (0: automatic)")
+    ap.add_argument("--xcd-run", type=int, default=0, help="accumulate 6: items per XCD run (0: automatic)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the HBM-counter calibration kernel) and exit")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "grid_kooshball3d_c32_traffic.txt"),
@@ -71,7 +73,8 @@ def copy_calibration():
 
 
 GRID_KERNELS = {1: "k_grid_items (LDS atomic add)", 2: "k_grid_items (LDS read-add-write)",
-                4: "k_grid_gather (cell-sorted batches, VGPR cells)", 5: "k_grid_mfma (matrix cores)"}
+                4: "k_grid_gather (cell-sorted batches, VGPR cells)", 5: "k_grid_mfma (matrix cores)",
+                6: "k_grid_pos (matrix cores, weights from sample positions)"}
 
 
 def read_traffic(path, workload, kernel):
@@ -310,7 +313,8 @@ def main():
     torch.cuda.empty_cache()
 
     t0 = time.perf_counter()
-    tune = {"accumulate": a.accumulate, "exact_order": 1 if a.exact else 0, "split_taps": a.split}
+    tune = {"accumulate": a.accumulate, "exact_order": 1 if a.exact else 0, "split_taps": a.split,
+            "items_per_wave": a.ipw, "xcd_run": a.xcd_run}
     if a.unit:
         tune.update(tile_x=a.unit[0], tile_y=a.unit[1], tile_z=a.unit[2])
     plan = hip.Plan(ctx, s, N, kw, lut, tune)
@@ -341,8 +345,8 @@ def main():
                 f"{C} coils, {a.image}^3 image, og 2 -> {N}^3 grid, KB width 4, L 512, complex64, DCF |k|^2")
     kernel = (GRID_KERNELS.get(info["accumulate"], "unknown")
               + f", unit {info['tile_x']}x{info['tile_y']}x{info['tile_z']}")
-    traffic, traffic_src = read_traffic(a.traffic_file, workload, kernel) if not (a.exact or a.unit or a.split
-                                                                                  or a.accumulate) else (None, None)
+    traffic, traffic_src = read_traffic(a.traffic_file, workload, kernel) if not (
+        a.exact or a.unit or a.split or a.accumulate or a.ipw or a.xcd_run) else (None, None)
     out = {
         "metric": "gridded samples/s, recon ms (3D radial, 32 coils); learned-DCF vs iterative",
         "value": C * M / (ms * 1e-3),

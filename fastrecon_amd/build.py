@@ -77,7 +77,7 @@ def build_cpp(verbose=False):
     lib_srcs = [s for s in srcs if not s.endswith("fastrecon_main.cpp")]
     tmp = API_LIB + ".tmp"
     _run(["g++", *CXXFLAGS, "-shared", "-o", tmp, *lib_srcs,
-          "-L", PKG, "-lfastrecon_hip", "-Wl,-rpath,$ORIGIN"])
+          "-L", PKG, "-lfastrecon_hip", "-lpthread", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, API_LIB)
     main = os.path.join(CPP, "src", "fastrecon_main.cpp")
     if os.path.exists(main):
@@ -88,9 +88,25 @@ def build_cpp(verbose=False):
     return API_LIB
 
 
+def build_test_programs(verbose=False):
+    """C++ programs the GPU tests run against the C++ API (tests/cpp/*.cpp ->
+    tests/cpp/<name>, linked to libfastrecon.so)."""
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    out = []
+    for src in sorted(glob.glob(os.path.join(tdir, "*.cpp"))):
+        exe = os.path.splitext(src)[0]
+        tmp = exe + ".tmp"
+        _run(["g++", *CXXFLAGS, "-o", tmp, src, "-L", PKG, "-lfastrecon", "-lfastrecon_hip", "-lpthread",
+              "-Wl,-rpath,$ORIGIN/../../fastrecon_amd"])
+        os.replace(tmp, exe)
+        out.append(exe)
+    return out
+
+
 def build_all(verbose=False):
     build_hip(verbose=verbose)
     build_cpp(verbose=verbose)
+    build_test_programs(verbose=verbose)
 
 
 if __name__ == "__main__":

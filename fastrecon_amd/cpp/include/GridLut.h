@@ -45,6 +45,14 @@ public:
     const fr_plan *devicePlan() const { return m_plan.get(); }
     unsigned gridSize() const { return m_gridSize; }
     const ConvKernel *kernel() const { return m_kernel; }
+    // true when the last plan() reused a device plan built for the same
+    // trajectory and parameters (by this or another GridLut)
+    bool planShared() const { return m_plan_shared; }
+    // Plans built by plan() in this process and still held (by a GridLut or
+    // as the most recent one, which the process keeps like the reference's
+    // static map, cuGridLut.h:50-56).
+    static int sharedPlanCount();
+    static void releaseSharedPlans();
 
 protected:
     ComplexVector<T> griddingChannel(int channel);
@@ -62,6 +70,12 @@ private:
     std::shared_ptr<fr_plan> m_plan;
     fr_tune m_tune{};
     bool m_has_tune = false;
+    bool m_plan_shared = false;
+    // last execute() result reused by griddingChannel while the data and
+    // the plan are unchanged
+    std::shared_ptr<ImageData<T>> m_channel_cache;
+    unsigned long long m_cache_version = 0;
+    const fr_plan *m_cache_plan = nullptr;
 };
 
 #endif // GRIDLUT_H

@@ -73,6 +73,9 @@ public:
     void invalidateDevice();
     // Replace the DCF from HBM (e.g. after an on-device DCF estimation).
     void refreshDcfFromDevice();
+    // Incremented by every change of trajectory, DCF or coil data (GridLut
+    // keys its cached gridding pass on it).
+    unsigned long long version() const { return m_version; }
 
 protected:
     int m_samples;
@@ -91,6 +94,7 @@ private:
 
     std::shared_ptr<fr_samples> m_dev;
     bool m_dev_traj = false, m_dev_dcf = false;
+    unsigned long long m_version = 0;
     std::vector<char> m_dev_coil;
 };
 #endif // RECONDATA_H

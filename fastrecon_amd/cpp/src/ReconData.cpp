@@ -40,6 +40,7 @@ void ReconData<T>::addChannelData(ComplexVector<T> &&data)
     }
     m_kDataMultiChannel.push_back(std::move(data));
     m_dev.reset();                                      // coil count changed
+    ++m_version;
 }
 
 template<typename T>
@@ -52,6 +53,7 @@ void ReconData<T>::setDcf(std::vector<T> &dcf)
     }
     m_dcf = std::move(dcf);
     m_dev_dcf = false;
+    ++m_version;
 }
 
 template<typename T>
@@ -78,6 +80,7 @@ void ReconData<T>::storeTrajComponent(const std::vector<T> &traj_c)
         p.x[m_dim] = *(in++);
     ++m_dim;
     m_dev_traj = false;
+    ++m_version;
 }
 
 template<typename T>
@@ -118,6 +121,7 @@ void ReconData<T>::loadFromFiles(const std::vector<std::string> &dataFileList,
         std::exit(1);
     }
     m_dev_dcf = false;
+    ++m_version;
 
     std::cout << std::endl << "Read data:" << std::endl;
     ComplexVector<T> kdata;
@@ -166,6 +170,7 @@ void ReconData<T>::loadTraj(const std::vector<T> &traj, int dim)
     }
     m_dev_traj = false;
     m_dev_dcf = false;
+    ++m_version;
 }
 
 #ifdef FASTRECON_WITH_QT
@@ -196,6 +201,7 @@ void ReconData<T>::updateSingleAcquisition(const std::complex<T> *data, int acqu
     }
     auto &channel_data = m_kDataMultiChannel[channel];
     std::copy_n(data, m_samples, channel_data.begin() + (std::size_t)m_samples * acquisition);
+    ++m_version;
     // cuReconData::updateSingleAcquisition (cuReconData.cu:52-62): keep the mirror current
     if (m_dev && channel < (int)m_dev_coil.size() && m_dev_coil[channel])
         check(fr_samples_update(m_dev.get(), channel, (int64_t)m_samples * acquisition, m_samples, data),
@@ -217,6 +223,7 @@ void ReconData<T>::transformLocalTraj(T translation, T scale)
         for (int comp = 0; comp < m_dim; ++comp)
             sample.x[comp] = (sample.x[comp] + translation) * scale;
     m_dev_traj = false;
+    ++m_version;
 }
 
 template<typename T>
@@ -243,6 +250,7 @@ void ReconData<T>::clear()
     m_kDataMultiChannel.clear();
     m_bounds.clear();
     m_dev.reset();
+    ++m_version;
 }
 
 template<typename T>
@@ -297,6 +305,7 @@ void ReconData<T>::refreshDcfFromDevice()
     m_dcf.resize(m_size);
     check(fr_samples_get_dcf(m_dev.get(), m_dcf.data()), "download dcf");
     m_dev_dcf = true;
+    ++m_version;
 }
 
 template class ReconData<float>;

@@ -90,14 +90,17 @@ extern "C" fr_status fr_samples_create(fr_ctx *ctx, int dim, int64_t M, int coil
     s->coil_set.assign(coils, 0);
     size_t esz = type == FR_SAMPLES_C64 ? 8 : 4;
     fr_status st;
+    // data and DCF hold one more sample, M, kept at zero: the null sample
+    // that pads the position-weight plans' record lists
     if ((st = fr_alloc(ctx, (void **)&s->traj, sizeof(float) * dim * M)) ||
-        (st = fr_alloc(ctx, (void **)&s->dcf, sizeof(float) * M)) ||
-        (st = fr_alloc(ctx, &s->data, esz * M * coils)) ||
+        (st = fr_alloc(ctx, (void **)&s->dcf, sizeof(float) * (M + 1))) ||
+        (st = fr_alloc(ctx, &s->data, esz * (M + 1) * coils)) ||
         (st = fr_alloc(ctx, &s->staging, 8 * M + 16 * M))) {
         fr_samples_destroy(s);
         return st;
     }
-    hipMemsetAsync(s->data, 0, esz * M * coils, ctx->stream);
+    hipMemsetAsync(s->data, 0, esz * (M + 1) * coils, ctx->stream);
+    hipMemsetAsync(s->dcf, 0, sizeof(float) * (M + 1), ctx->stream);
     *out = s;
     return FR_OK;
 }
@@ -139,8 +142,8 @@ extern "C" fr_status fr_samples_create_frames(fr_ctx *ctx, int frames, const int
     int64_t *off_d = nullptr;
     fr_status st;
     if ((st = fr_alloc(ctx, (void **)&s->traj, sizeof(float) * 3 * M)) ||
-        (st = fr_alloc(ctx, (void **)&s->dcf, sizeof(float) * M)) ||
-        (st = fr_alloc(ctx, &s->data, esz * M * coils)) ||
+        (st = fr_alloc(ctx, (void **)&s->dcf, sizeof(float) * (M + 1))) ||
+        (st = fr_alloc(ctx, &s->data, esz * (M + 1) * coils)) ||
         (st = fr_alloc(ctx, &s->staging, 8 * M + 16 * M)) ||
         (st = fr_alloc(ctx, (void **)&off_d, sizeof(int64_t) * (frames + 1)))) {
         hipFree(off_d);
@@ -152,8 +155,9 @@ extern "C" fr_status fr_samples_create_frames(fr_ctx *ctx, int frames, const int
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_frame_rows, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, ctx->stream,
                            s->traj + 2 * M, off_d, frames, M);
-        e = hipMemsetAsync(s->data, 0, esz * M * coils, ctx->stream);
+        e = hipMemsetAsync(s->data, 0, esz * (M + 1) * coils, ctx->stream);
     }
+    if (e == hipSuccess) e = hipMemsetAsync(s->dcf, 0, sizeof(float) * (M + 1), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(off_d);
     if (e != hipSuccess) {

@@ -175,7 +175,10 @@ extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int
             // times faster here than the LDS kernels, whose lanes are the coil
             // components; an exact-order plan keeps its sequential mode (CPU
             // summation order)
-            st = fr_launch_grid(p, u, g->data, p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
+            // (a position-weight plan stores no taps: it keeps its own mode)
+            const int mode = p->accumulate == FR_ACC_POS ? FR_ACC_POS
+                             : (p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
+            st = fr_launch_grid(p, u, g->data, mode);
             if (!st) st = fr_degrid_raw(p, u, g->data, 1, den);
             if (!st) hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, den, M);
         }

@@ -490,6 +490,9 @@ fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force
     case FR_ACC_MFMA:
         e = fr_launch_grid_mfma(p, s, out);
         break;
+    case FR_ACC_POS:
+        e = fr_launch_grid_pos(p, s, out);
+        break;
     case FR_ACC_GATHER:
         e = c64 ? launch<FR_ACC_GATHER, float2>(p, s, out) : launch<FR_ACC_GATHER, __half2>(p, s, out);
         break;

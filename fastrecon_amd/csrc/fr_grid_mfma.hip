@@ -198,7 +198,11 @@ __device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st
         const int rec = 4 * q + k;
         // B = dcf * sample (GridLut.cpp:109); padding coil columns and
         // records past the item's end get 0
-        const float w = rb + (uint32_t)rec < r1 ? S.w[q] : 0.f;
+        // (a record past the item's end is another unit's sample: its B row
+        // is selected to 0, not multiplied by 0, so a NaN or Inf there
+        // cannot reach this unit)
+        const bool live = rb + (uint32_t)rec < r1;
+        const float w = S.w[q];
         const float w0 = FULL || v0ok ? w : 0.f;
         float bv[NCB];
         if constexpr (NCB == 4) {
@@ -210,6 +214,8 @@ __device__ __forceinline__ void apply_batch(f4 (&acc)[NBLK][NCB], const batch_st
         } else {
             bv[0] = __fmul_rn(w0, (j & 1) ? S.a[q].y : S.a[q].x);
         }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) bv[cb] = live ? bv[cb] : 0.f;
 #pragma unroll
         for (int b = 0; b < NBLK; ++b) {
             const float aw = tile[rec * ROW + 16 * b + j];

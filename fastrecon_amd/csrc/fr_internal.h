@@ -62,6 +62,10 @@ struct fr_grids {
 #define FR_GATHER_BATCH 32  // records per gather batch
 #define FR_GATHER_CELLS 32  // unit cells held in VGPRs by the gather kernel
 #define FR_ACC_MFMA 5       // matrix-core products of 16-cell blocks (weights from the plan's taps)
+#define FR_ACC_POS 6        // matrix-core products of 16-cell units, weights computed from sample positions
+#define FR_POS_MAX_L 2048   // longest LUT of a position-weight plan (its selection table is staged in LDS)
+#define FR_POS_WAVES 4      // waves per workgroup of the position-weight kernel
+#define FR_POS_BATCH 16     // records per batch of the position-weight kernel; units are padded to it
 #define FR_MFMA_MAX_L 4096  // LUT lengths for which the plan builds exact index thresholds (degridding)
 #define FR_CT_MAX_L 1023    // longest LUT for compact 2-byte taps (10-bit index, index L = zero weight)
 #ifndef FR_MFMA_XCD_RUN
@@ -106,6 +110,16 @@ struct fr_item {
     int32_t slot;                // -1: write the grid directly
     uint32_t r0, r1, t0;         // records [r0, r1), first tap t0
     uint32_t pad;
+};
+
+// Position-weight mode: a tile is FR_POS_WAVES x-adjacent units gridded by
+// one workgroup (one wave each) and written out together in whole rows;
+// item[w] is the item of unit w, or -1 when that unit is written elsewhere
+// (front units) -- its cells are then left alone.  The tile's box: box0 =
+// ox | oy << 16, box1 = oz.
+struct fr_tile {
+    int32_t item[4];
+    uint32_t box0, box1;
 };
 
 // A heavy unit: its items are [item0, item0 + nitems), partial slots
@@ -154,6 +168,24 @@ struct fr_plan {
     float mfma_dacc = 0.f;             // least d2 with sqrtf(d2) >= hw (rejected from here on)
     float mfma_cidx = 0.f;             // approximately (L - 1) / hw
     int64_t nbatches = 0;
+    // position-weight mode (FR_ACC_POS): no taps.  hdrs is unused; recs holds
+    // each unit's sample indices in ascending order, padded with the index M
+    // to a multiple of FR_POS_BATCH (every unit starts at such a multiple),
+    // then 64 more entries M; pos[M + 1] the samples' grid positions (x, y,
+    // z, 0), entry M far outside the grid; tsel[L] the LUT-index selection
+    // table (B[m+1], LUT[m], LUT[m+1], 0) for the estimate m
+    // (fr_plan_pos.hip).  Items carry their unit's box: t0 = ox | oy << 16,
+    // pad = oz | (ux | uy << 5 | uz << 10) << 16.
+    uint32_t *recs = nullptr;
+    float4 *pos = nullptr;
+    float4 *tsel = nullptr;
+    int pos_ipw = 1;                   // items per wave (light items)
+    int pos_stores = 2;                // grid stores: 1 streaming, 2 cached
+    int pos_waves = 4;                 // kernel variant: 3 or 4 waves per SIMD
+    int xcd_run = FR_MFMA_XCD_RUN;     // items per XCD run in launch order
+    int64_t nrecs_padded = 0;
+    fr_tile *tiles = nullptr;          // position-weight mode: light units by tile, launch order
+    int64_t ntiles = 0;                // 0: light items are written unit by unit
     double build_ms = 0.0;
 };
 
@@ -207,6 +239,7 @@ static inline size_t fr_plan_cells(int dim, unsigned N, int frames)
 // records serve FR_ACC_MFMA when its LUT has at most FR_MFMA_MAX_L entries)
 fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode);
 fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out);
+fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out);
 // cropped FFT + SOS helpers (fr_fftcrop.hip)
 fr_status fr_fft_workspace(fr_fft *f, size_t bytes);
 // FR_ERR_INVALID unless [p, p + bytes) lies inside the device allocation of p
@@ -219,3 +252,7 @@ fr_status fr_check_device_extent(fr_ctx *ctx, const void *p, size_t bytes, const
 fr_status fr_fft_tw(fr_fft *f);
 bool fr_fft_sos_fused(const fr_fft *f, const fr_grids *g, unsigned rcx, unsigned rcy, unsigned rcz);
 fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune);
+fr_status fr_plan_build_pos(fr_plan *p, const fr_samples *s, const fr_tune *tune);
+// exact LUT-index thresholds of a plan (mfma_pairs, mfma_dacc, mfma_cidx) and,
+// when sel is set, the position-weight selection table (tsel)
+fr_status fr_plan_thresholds(fr_plan *p, bool sel);

@@ -549,31 +549,7 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     gctx g{p->kHW, p->L, p->dim, N, p->frames};
     fr_status e;
 
-    if (p->L <= FR_MFMA_MAX_L) {
-        // (every plan: the matrix-core kernel also serves single-channel
-        // density gridding and degridding uses the same thresholds)
-        // The matrix-core kernel takes a fast (1 ulp) square root, whose
-        // index estimate is within one of the reference index, and corrects
-        // it exactly with these thresholds: index = m where B[m] <= d2 <
-        // B[m+1].  The index is monotone in d2, so the thresholds exist.
-        const float hw = p->kHW;
-        const int L = p->L;
-        const float top = 64.f * hw * hw + 64.f;        // far beyond every accepted tap
-        uint32_t top_bits;
-        std::memcpy(&top_bits, &top, 4);
-        std::vector<float> B(L + 2, INFINITY);
-        for (int m = 0; m < L; ++m) {
-            if (ref_lut_index(top, hw, L) < m) break;           // not reached below the search bound
-            B[m] = least_float(top_bits, [&](float d2) { return ref_lut_index(d2, hw, L) >= m; });
-        }
-        p->mfma_dacc = least_float(top_bits, [&](float d2) { return sqrtf(d2) >= hw; });
-        p->mfma_cidx = (float)((double)(L - 1) / (double)hw);
-        std::vector<float2> pairs(L + 1);
-        for (int m = 0; m <= L; ++m) pairs[m] = make_float2(B[m], B[m + 1]);
-        if ((e = fr_alloc(ctx, (void **)&p->mfma_pairs, sizeof(float2) * (L + 1)))) return e;
-        FR_HIP(ctx, hipMemcpyAsync(p->mfma_pairs, pairs.data(), sizeof(float2) * (L + 1), hipMemcpyHostToDevice, st));
-        FR_HIP(ctx, hipStreamSynchronize(st));
-    }
+    if (p->L <= FR_MFMA_MAX_L && (e = fr_plan_thresholds(p, false))) return e;
 
     // 1. per-cell tap histogram
     dbuf<uint32_t> cnt_d;
@@ -866,6 +842,66 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune)
     return FR_OK;
 }
 
+
+// Exact LUT-index thresholds for a fast (1 ulp) square root.  The reference
+// index roundf(sqrtf(d2) / hw * (L - 1)) (GridLut.cpp:112-136) is monotone in
+// the squared distance d2, so B[m] = least d2 whose index is >= m exists, and
+// the index of d2 is the m with B[m] <= d2 < B[m + 1].  B is found by
+// bisection over float bit patterns with the reference arithmetic on the
+// host.  mfma_pairs[m] = (B[m], B[m + 1]) serves the degridding kernel
+// (estimate within one of the index, corrected both ways); with sel,
+// tsel[m] = (B[m + 1], LUT[m], LUT[m + 1], 0) serves the position-weight
+// gridding kernel (estimate the index or one below it).
+fr_status fr_plan_thresholds(fr_plan *p, bool sel)
+{
+    fr_ctx *ctx = p->ctx;
+    hipStream_t st = ctx->stream;
+    fr_status e;
+    const float hw = p->kHW;
+    const int L = p->L;
+    const float top = 64.f * hw * hw + 64.f;        // far beyond every accepted tap
+    uint32_t top_bits;
+    std::memcpy(&top_bits, &top, 4);
+    std::vector<float> B(L + 2, INFINITY);
+    for (int m = 0; m < L; ++m) {
+        if (ref_lut_index(top, hw, L) < m) break;           // not reached below the search bound
+        B[m] = least_float(top_bits, [&](float d2) { return ref_lut_index(d2, hw, L) >= m; });
+    }
+    p->mfma_dacc = least_float(top_bits, [&](float d2) { return sqrtf(d2) >= hw; });
+    p->mfma_cidx = (float)((double)(L - 1) / (double)hw);
+    std::vector<float2> pairs(L + 1);
+    for (int m = 0; m <= L; ++m) pairs[m] = make_float2(B[m], B[m + 1]);
+    if (!p->mfma_pairs && (e = fr_alloc(ctx, (void **)&p->mfma_pairs, sizeof(float2) * (L + 1)))) return e;
+    FR_HIP(ctx, hipMemcpyAsync(p->mfma_pairs, pairs.data(), sizeof(float2) * (L + 1), hipMemcpyHostToDevice, st));
+    if (sel) {
+        std::vector<float4> t(L);
+        for (int m = 0; m < L; ++m)
+            t[m] = make_float4(B[m + 1], p->host_lut[m], m + 1 < L ? p->host_lut[m + 1] : 0.f, 0.f);
+        if (!p->tsel && (e = fr_alloc(ctx, (void **)&p->tsel, sizeof(float4) * L))) return e;
+        FR_HIP(ctx, hipMemcpyAsync(p->tsel, t.data(), sizeof(float4) * L, hipMemcpyHostToDevice, st));
+    }
+    FR_HIP(ctx, hipStreamSynchronize(st));
+    return FR_OK;
+}
+
+// The position-weight matrix-core mode (accumulate 6) is the default for
+// plans that are not exact-order, unless the LUT is too long for its LDS
+// table or the tuned unit box exceeds 16 cells (then accumulate 5).
+static bool fr_plan_uses_pos(const fr_plan *p, const fr_samples *s, const fr_tune *tune)
+{
+    (void)s;
+    const int acc = tune ? tune->accumulate : 0;
+    if (acc == FR_ACC_POS) return true;
+    if (acc != 0 || (tune && tune->exact_order) || p->L > FR_POS_MAX_L) return false;
+    if (tune) {
+        const int bx = tune->tile_x > 0 ? tune->tile_x : 4;
+        const int by = tune->tile_y > 0 ? tune->tile_y : (p->dim == 2 ? 4 : 2);
+        const int bz = p->dim == 3 && tune->tile_z > 0 ? tune->tile_z : (p->dim == 2 ? 1 : 2);
+        if (bx * by * bz > 16) return false;
+    }
+    return true;
+}
+
 extern "C" fr_status fr_plan_create(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth,
                                     const float *lut, int L, const fr_tune *tune, fr_plan **out)
 {
@@ -889,7 +925,7 @@ extern "C" fr_status fr_plan_create(fr_ctx *ctx, const fr_samples *s, unsigned N
         hipError_t he = hipMemcpyAsync(p->lut, lut, sizeof(float) * L, hipMemcpyHostToDevice, ctx->stream);
         if (he != hipSuccess) e = fr_fail(ctx, FR_ERR_HIP, "lut upload: %s", hipGetErrorString(he));
     }
-    if (!e) e = fr_plan_build(p, s, tune);
+    if (!e) e = fr_plan_uses_pos(p, s, tune) ? fr_plan_build_pos(p, s, tune) : fr_plan_build(p, s, tune);
     if (e) { fr_plan_destroy(p); return e; }
     *out = p;
     return FR_OK;
@@ -901,6 +937,7 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
     hipStreamSynchronize(p->ctx->stream);
     hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps); hipFree(p->tstart);
     hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches); hipFree(p->mfma_pairs);
+    hipFree(p->recs); hipFree(p->pos); hipFree(p->tsel); hipFree(p->tiles);
     delete p;
     return FR_OK;
 }
@@ -916,9 +953,13 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->slabs = p->geom.waves;
     info->tiles = (int)std::min<int64_t>(p->nunits, INT32_MAX);
     info->lds_bytes = 0;
-    info->plan_bytes = (int64_t)(sizeof(uint2) + sizeof(uint32_t)) * p->nrecords +
-                       (int64_t)(p->compact_taps ? sizeof(uint16_t) : sizeof(uint2)) * p->ntaps +
-                       (int64_t)sizeof(fr_unit) * p->nunits;
+    if (p->accumulate == FR_ACC_POS)
+        info->plan_bytes = (int64_t)sizeof(uint32_t) * (p->nrecs_padded + 64) + (int64_t)sizeof(float4) * (p->M + 1) +
+                           (int64_t)sizeof(fr_unit) * p->nunits + (int64_t)sizeof(fr_item) * p->nitems;
+    else
+        info->plan_bytes = (int64_t)(sizeof(uint2) + sizeof(uint32_t)) * p->nrecords +
+                           (int64_t)(p->compact_taps ? sizeof(uint16_t) : sizeof(uint2)) * p->ntaps +
+                           (int64_t)sizeof(fr_unit) * p->nunits;
     info->build_ms = p->build_ms;
     info->items = p->nitems;
     info->heavy_units = p->nheavy;

@@ -50,7 +50,8 @@ class HipError(RuntimeError):
 class fr_tune(C.Structure):
     _fields_ = [("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
                 ("accumulate", C.c_int), ("split_taps", C.c_int),
-                ("exact_order", C.c_int)]
+                ("exact_order", C.c_int), ("items_per_wave", C.c_int), ("xcd_run", C.c_int),
+                ("stores", C.c_int), ("waves_per_simd", C.c_int), ("tile_writes", C.c_int)]
 
 
 class fr_plan_info(C.Structure):

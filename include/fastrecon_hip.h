@@ -71,13 +71,34 @@ typedef struct {
                                        v_mfma_f32_16x16x4_f32, weights from the
                                        plan's taps (reference tap set and LUT
                                        index); sums group four products, so
-                                       exact_order is refused */
-    int split_taps;              /* heavier units are split (0: automatic)      */
+                                       exact_order is refused,
+                                    6: matrix cores, weights computed from the
+                                       sample positions with the reference tap
+                                       test and LUT index (no taps stored; unit
+                                       box of at most 16 cells, LUT of at most
+                                       2048 entries); the automatic choice when
+                                       these hold; exact_order is refused */
+    int split_taps;              /* heavier units are split (0: automatic); for
+                                    accumulate 6 the unit is records, not taps:
+                                    a unit's records are cut into slices of at
+                                    most this many (rounded up to 16)          */
     int exact_order;             /* 1: split units by space only, so every cell
                                     sums in ascending sample order (bit-identical
                                     to the CPU path); 0: heavy units are also split
                                     by sample range into slices combined in a fixed
                                     order (deterministic, not bitwise CPU order) */
+    int items_per_wave;          /* accumulate 6: light items one wave grids in
+                                    turn (0: 1)                                 */
+    int xcd_run;                 /* accumulate 6: consecutive items each of the 8
+                                    XCDs takes in launch order (0: 256)         */
+    int stores;                  /* accumulate 6: grid stores 0: automatic
+                                    (cached), 1: streaming (nontemporal), 2: cached */
+    int waves_per_simd;          /* accumulate 6: 0: automatic (4), 3 or 4 (the
+                                    kernel variant compiled for that occupancy) */
+    int tile_writes;             /* accumulate 6: 0: automatic (off), 1: four
+                                    x-adjacent units per workgroup written out
+                                    together in whole rows, 2: off (each unit
+                                    written by its wave)                        */
 } fr_tune;
 
 typedef struct {
@@ -87,7 +108,7 @@ typedef struct {
     double build_ms;
     int64_t items, heavy_units, max_unit_taps;
     int exact_order;
-    int accumulate;              /* the plan's resolved accumulation mode (1, 2, 4 or 5) */
+    int accumulate;              /* the plan's resolved accumulation mode (1, 2, 4, 5 or 6) */
 } fr_plan_info;
 
 /* ------------------------------------------------------------ host helpers */

@@ -162,7 +162,7 @@ def test_autotune_picks_a_working_candidate(ctx, orc):
     assert ms == pytest.approx(cand[cand > 0].min())
     # the winner grids correctly; sequential modes also in exact CPU order
     # (the matrix-core mode refuses exact_order: it sums four products at a time)
-    exact = best["accumulate"] != 5
+    exact = best["accumulate"] not in (5, 6)
     p = hip.Plan(ctx, s, N, 4.0, lut, dict(best, exact_order=1) if exact else best)
     g = hip.Grids(ctx, 2, N, C, frames=len(tns))
     hip.grid_execute(p, s, g)

@@ -1,6 +1,8 @@
-"""GPU parity of the matrix-core gridding mode (accumulate = 5) through the C ABI.
+"""GPU parity of the matrix-core gridding modes through the C ABI: accumulate
+5 (weights from the plan's taps) and 6 (weights computed from the sample
+positions, the default).
 
-The matrix-core kernel computes every tap weight with the reference arithmetic
+Both kernels use every tap weight of the reference arithmetic
 (tap test sqrtf(d2) < hw, LUT index roundf(dk / hw * (L - 1)), GridLut.cpp:99-
 136) and multiplies it with dcf * sample (GridLut.cpp:109) like the CPU loop;
 only the order of the sum over samples differs (four products per matrix
@@ -21,7 +23,23 @@ from fastrecon_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
-MFMA = 5
+
+
+@pytest.fixture(params=[5, 6], ids=["taps", "positions"])
+def mode(request):
+    return request.param
+
+
+def cells_of(tune, dim):
+    d = dict(tune or {})
+    bx, by = d.get("tile_x", 4), d.get("tile_y", 4 if dim == 2 else 2)
+    bz = d.get("tile_z", 2) if dim == 3 else 1
+    return bx * by * bz
+
+
+def skip_big_units(mode, tune, dim):
+    if mode == 6 and cells_of(tune, dim) > 16:
+        pytest.skip("the position-weight mode takes unit boxes of at most 16 cells")
 
 
 @pytest.fixture(scope="module")
@@ -35,14 +53,14 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-30))
 
 
-def run_grid(ctx, traj_n, dcf, data, N, lut, kwidth=4.0, tune=None, sample_type=hip.FR_SAMPLES_C64):
+def run_grid(ctx, traj_n, dcf, data, N, lut, kwidth=4.0, tune=None, sample_type=hip.FR_SAMPLES_C64, mode=5):
     M, dim = traj_n.shape
     C = data.shape[0]
     s = hip.Samples(ctx, dim, M, C, sample_type)
     s.set_traj(traj_n)
     s.set_dcf(dcf)
     s.set_coils(data)
-    p = hip.Plan(ctx, s, N, kwidth, lut, dict(tune or {}, accumulate=MFMA))
+    p = hip.Plan(ctx, s, N, kwidth, lut, dict(tune or {}, accumulate=mode))
     g = hip.Grids(ctx, dim, N, C)
     hip.grid_execute(p, s, g)
     out = g.download_all()
@@ -84,18 +102,22 @@ def sparse_points(dim, N, spacing, rng, jitter=1.0):
     (3, 32, {}), (3, 8, {}), (3, 12, {}), (3, 1, {}), (3, 40, {}),
     (3, 32, dict(tile_x=4, tile_y=4, tile_z=2)), (3, 32, dict(tile_x=4, tile_y=4, tile_z=4)),
     (3, 16, dict(tile_x=3, tile_y=3, tile_z=3)), (3, 32, dict(tile_x=8, tile_y=2, tile_z=1)),
-    (2, 8, {}), (2, 12, dict(tile_x=8, tile_y=4)), (2, 32, dict(tile_x=8, tile_y=8))])
-def test_sparse_samples_bitwise(ctx, orc, dim, coils, tune):
+    (3, 32, dict(tile_x=2, tile_y=2, tile_z=4)), (3, 8, dict(tile_x=4, tile_y=4, tile_z=1)),
+    (3, 32, dict(tile_writes=1)), (3, 16, dict(tile_writes=1, stores=1)),
+    (2, 8, {}), (2, 12, dict(tile_x=8, tile_y=4)), (2, 32, dict(tile_x=8, tile_y=8)), (2, 16, dict(tile_x=8, tile_y=2)),
+    (2, 8, dict(tile_writes=1))])
+def test_sparse_samples_bitwise(ctx, orc, mode, dim, coils, tune):
+    skip_big_units(mode, tune, dim)
     rng = np.random.default_rng(11 + dim + coils)
     N = 48 if dim == 3 else 96
     pts = sparse_points(dim, N, 5.0, rng)
     dcf = rng.uniform(0.1, 2.0, size=len(pts)).astype(np.float32)
     data = syn.coil_data(coils, len(pts), 21)
     lut, _ = hip.kb_lut(4.0, 2.0, 512)
-    got, info = run_grid(ctx, pts, dcf, data, N, lut, tune=tune)
+    got, info = run_grid(ctx, pts, dcf, data, N, lut, tune=tune, mode=mode)
     ref = orc.grid_f32(pts, dcf, data, N, 4.0, lut, nthreads=8)
     same = np.array_equal(got.view(np.int32), ref.view(np.int32))
-    print(f"[sparse {dim}D C={coils} {tune}] M={len(pts)} tile={info['tile_x']}x{info['tile_y']}x{info['tile_z']} "
+    print(f"[sparse mode {mode} {dim}D C={coils} {tune}] M={len(pts)} tile={info['tile_x']}x{info['tile_y']}x{info['tile_z']} "
           f"rel-L2 {rel_l2(got, ref):.2e} bitwise {same}", flush=True)
     np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
 
@@ -104,35 +126,43 @@ def test_sparse_samples_bitwise(ctx, orc, dim, coils, tune):
                                         (34, {}),
                                         (32, dict(split_taps=300)), (32, dict(tile_x=4, tile_y=4, tile_z=2)),
                                         (32, dict(tile_x=4, tile_y=4, tile_z=4, split_taps=2000)),
-                                        (12, dict(tile_x=3, tile_y=3, tile_z=3))])
-def test_kooshball_parity_and_determinism(ctx, orc, coils, tune):
+                                        (12, dict(tile_x=3, tile_y=3, tile_z=3)),
+                                        (32, dict(split_taps=16)), (32, dict(items_per_wave=3, xcd_run=40)),
+                                        (32, dict(tile_writes=1)), (12, dict(tile_writes=1, split_taps=64)),
+                                        (32, dict(tile_writes=1, stores=1, waves_per_simd=3)),
+                                        (8, dict(tile_writes=1, tile_x=4, tile_y=4, tile_z=1)),
+                                        (32, dict(stores=1, waves_per_simd=3, items_per_wave=2)),
+                                        (32, dict(tile_x=8, tile_y=2, tile_z=1, items_per_wave=2)),
+                                        (16, dict(tile_x=2, tile_y=2, tile_z=4, split_taps=100))])
+def test_kooshball_parity_and_determinism(ctx, orc, mode, coils, tune):
+    skip_big_units(mode, tune, 3)
     tn, dcf, data, N, lut = case("kooshball3d", 900, 64, coils, seed=6, image=32)
-    a, info = run_grid(ctx, tn, dcf, data, N, lut, tune=tune)
-    b, _ = run_grid(ctx, tn, dcf, data, N, lut, tune=tune)
+    a, info = run_grid(ctx, tn, dcf, data, N, lut, tune=tune, mode=mode)
+    b, _ = run_grid(ctx, tn, dcf, data, N, lut, tune=tune, mode=mode)
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
     ref = orc.grid_f32(tn, dcf, data, N, 4.0, lut, nthreads=8)
     ref64 = orc.grid_f64(tn, dcf, data, N, 4.0, lut, nthreads=8)
     e32, e64, c64 = rel_l2(a, ref), rel_l2(a, ref64), rel_l2(ref, ref64)
-    print(f"[mfma C={coils} {tune}] heavy={info['heavy_units']} items={info['items']} "
+    print(f"[mode {mode} C={coils} {tune}] heavy={info['heavy_units']} items={info['items']} "
           f"rel-L2 vs CPU-FP32 {e32:.2e}, vs FP64 {e64:.2e} (CPU-FP32 vs FP64 {c64:.2e})", flush=True)
     assert e32 <= TOL
     assert e64 <= 2 * c64 + 1e-7
 
 
-def test_radial2d_full_config(ctx, orc):
+def test_radial2d_full_config(ctx, orc, mode):
     """Config-1 geometry: 402 golden-angle spokes x 512, 8 coils, 512^2 grid."""
     tn, dcf, data, N, lut = case("radial2d", 402, 512, 8, seed=1)
-    got, info = run_grid(ctx, tn, dcf, data, N, lut)
+    got, info = run_grid(ctx, tn, dcf, data, N, lut, mode=mode)
     ref = orc.grid_f32(tn, dcf, data, N, 4.0, lut, nthreads=8)
     ref64 = orc.grid_f64(tn, dcf, data, N, 4.0, lut, nthreads=8)
     e32, e64, c64 = rel_l2(got, ref), rel_l2(got, ref64), rel_l2(ref, ref64)
-    print(f"[mfma radial2d] M={tn.shape[0]} N={N} items={info['items']} rel-L2 vs CPU-FP32 {e32:.2e}, "
+    print(f"[mode {mode} radial2d] M={tn.shape[0]} N={N} items={info['items']} rel-L2 vs CPU-FP32 {e32:.2e}, "
           f"vs FP64 {e64:.2e} (CPU-FP32 vs FP64 {c64:.2e})", flush=True)
     assert e32 <= TOL
     assert e64 <= 2 * c64 + 1e-7
 
 
-def test_edges_integer_centres_and_outside_samples(ctx, orc):
+def test_edges_integer_centres_and_outside_samples(ctx, orc, mode):
     rng = np.random.default_rng(3)
     N = 40
     pts = rng.uniform(-3.0, N + 2.0, size=(3000, 3)).astype(np.float32)
@@ -142,14 +172,14 @@ def test_edges_integer_centres_and_outside_samples(ctx, orc):
     dcf = rng.uniform(0, 2, size=len(pts)).astype(np.float32)
     data = syn.coil_data(3, len(pts), 9)
     lut, _ = hip.kb_lut(4.0, 2.0, 512)
-    got, _ = run_grid(ctx, pts, dcf, data, N, lut)
+    got, _ = run_grid(ctx, pts, dcf, data, N, lut, mode=mode)
     ref = orc.grid_f32(pts, dcf, data, N, 4.0, lut)
-    print(f"[mfma edges] rel-L2 {rel_l2(got, ref):.2e}", flush=True)
+    print(f"[mode {mode} edges] rel-L2 {rel_l2(got, ref):.2e}", flush=True)
     assert rel_l2(got, ref) <= TOL
 
 
 @pytest.mark.parametrize("kw,og,L", [(3.0, 2.0, 512), (5.0, 1.5, 256), (4.0, 1.5, 1024)])
-def test_other_kernel_widths_and_lut_lengths(ctx, orc, kw, og, L):
+def test_other_kernel_widths_and_lut_lengths(ctx, orc, mode, kw, og, L):
     """hw that is not a power of two (1.5, 2.5) and other LUT lengths: the
     exact index thresholds are built from the reference arithmetic."""
     rng = np.random.default_rng(int(kw * 10 + L))
@@ -158,26 +188,74 @@ def test_other_kernel_widths_and_lut_lengths(ctx, orc, kw, og, L):
     dcf = rng.uniform(0.1, 2.0, size=len(pts)).astype(np.float32)
     data = syn.coil_data(8, len(pts), 4)
     lut, _ = hip.kb_lut(kw, og, L)
-    got, _ = run_grid(ctx, pts, dcf, data, N, lut, kwidth=kw)
+    got, _ = run_grid(ctx, pts, dcf, data, N, lut, kwidth=kw, mode=mode)
     ref = orc.grid_f32(pts, dcf, data, N, kw, lut)
     np.testing.assert_array_equal(got.view(np.int32), ref.view(np.int32))
 
 
-def test_fp16_samples(ctx, orc):
+def test_fp16_samples(ctx, orc, mode):
     tn, dcf, data, N, lut = case("kooshball3d_fp16", 500, 96, 16, seed=3, image=48)
-    got, _ = run_grid(ctx, tn, dcf, data, N, lut, sample_type=hip.FR_SAMPLES_C16)
+    got, _ = run_grid(ctx, tn, dcf, data, N, lut, sample_type=hip.FR_SAMPLES_C16, mode=mode)
     deq = (data.real.astype(np.float16).astype(np.float32)
            + 1j * data.imag.astype(np.float16).astype(np.float32)).astype(np.complex64)
     ref = orc.grid_f32(tn, dcf, deq, N, 4.0, lut, nthreads=8)
     ref64 = orc.grid_f64(tn, dcf, deq, N, 4.0, lut, nthreads=8)
     e = [rel_l2(got[c], ref64[c]) for c in range(16)]
-    print(f"[mfma fp16] rel-L2 vs CPU-FP32 {rel_l2(got, ref):.2e}; per coil vs FP64:",
+    print(f"[mode {mode} fp16] rel-L2 vs CPU-FP32 {rel_l2(got, ref):.2e}; per coil vs FP64:",
           " ".join(f"{x:.2e}" for x in e), flush=True)
     assert rel_l2(got, ref) <= TOL
     assert max(e) < 1e-6
 
 
-def test_exact_order_is_refused(ctx, orc):
+def test_exact_order_is_refused(ctx, orc, mode):
     tn, dcf, data, N, lut = case("kooshball3d", 50, 32, 2, seed=5, image=16)
     with pytest.raises(hip.HipError, match="exact_order"):
-        run_grid(ctx, tn, dcf, data, N, lut, tune=dict(exact_order=1))
+        run_grid(ctx, tn, dcf, data, N, lut, tune=dict(exact_order=1), mode=mode)
+
+
+def test_position_mode_refuses_big_units_and_long_luts(ctx, orc):
+    tn, dcf, data, N, lut = case("kooshball3d", 50, 32, 2, seed=5, image=16)
+    with pytest.raises(hip.HipError, match="16 cells"):
+        run_grid(ctx, tn, dcf, data, N, lut, tune=dict(tile_x=8, tile_y=2, tile_z=2), mode=6)
+    lut4k, _ = hip.kb_lut(4.0, 2.0, 4096)
+    with pytest.raises(hip.HipError, match="2048"):
+        run_grid(ctx, tn, dcf, data, N, lut4k, mode=6)
+
+
+def test_default_plan_is_position_mode(ctx, orc):
+    tn, dcf, data, N, lut = case("kooshball3d", 60, 32, 4, seed=5, image=16)
+    s = hip.Samples(ctx, 3, tn.shape[0], 4)
+    s.set_traj(tn)
+    p = hip.Plan(ctx, s, N, 4.0, lut)
+    assert p.info()["accumulate"] == 6
+    assert (p.info()["tile_x"], p.info()["tile_y"], p.info()["tile_z"]) == (4, 2, 2)
+    p8 = hip.Plan(ctx, s, N, 4.0, lut, dict(tile_x=8, tile_y=2, tile_z=2))
+    assert p8.info()["accumulate"] == 5
+    for h in (p8, p, s):
+        h.close()
+
+
+def test_nan_samples_stay_in_the_units_they_touch(ctx, orc, mode):
+    """Documented deviation of the matrix-core modes: a NaN sample value
+    reaches every cell of the units its taps touch (0 * NaN in the matrix
+    product), but no cell outside them, and other coils are unaffected."""
+    rng = np.random.default_rng(8)
+    N = 32
+    pts = sparse_points(3, N, 9.0, rng)
+    dcf = np.ones(len(pts), np.float32)
+    data = syn.coil_data(2, len(pts), 4)
+    data[0, 5] = np.nan
+    lut, _ = hip.kb_lut(4.0, 2.0, 512)
+    got, info = run_grid(ctx, pts, dcf, data, N, lut, mode=mode)
+    ref = orc.grid_f32(pts, dcf, data, N, 4.0, lut)
+    bad = ~np.isfinite(got[0])
+    refbad = ~np.isfinite(ref[0])
+    assert refbad.sum() > 0 and np.all(bad[refbad])
+    # every non-finite cell lies within 4 cells of the NaN sample (its
+    # footprint widened to whole units)
+    cz, cy, cx = np.unravel_index(np.nonzero(bad)[0], (N, N, N))
+    d = np.maximum.reduce([np.abs(cx - pts[5, 0]), np.abs(cy - pts[5, 1]), np.abs(cz - pts[5, 2])])
+    assert d.max() < 2 + 8
+    fin = ~bad
+    assert rel_l2(got[0][fin], ref[0][fin]) <= TOL
+    assert rel_l2(got[1], ref[1]) <= TOL
