@@ -6,14 +6,16 @@ Fibonacci-sphere directions x 256 samples (13,107,200 samples per coil),
 oversampling 2 (128^3 image -> 256^3 grid), 512-entry LUT, FP32
 accumulation.  Inputs are resident in HBM before the timed region.
 
-A step is one gridding pass over all 32 coils (one kernel launch).  `value`
+A step is one gridding pass over all 32 coils (fr_grid_execute).  `value`
 is gridded coil-samples per second = C * M / t_step.  The JSON line also
-carries the roofline of the gridding kernel (algorithmic bytes
-C*M*8 + M*(4*3+4) + C*N^3*8 per launch over its average duration measured
+carries the roofline of the gridding pass (algorithmic bytes
+C*M*8 + M*(4*3+4) + C*N^3*8 per pass over its average duration measured
 with HIP events on the launch stream), the full reconstruction latency
-(grid + batched FFT + fused shift/crop/SOS), the plan build time, and a CPU
-baseline: the oracle's FP32 restatement of the reference CPU gridding loop
-timed on this host's cores on a bounded slice of the same workload.
+(grid + cropped FFT + fused shift/crop/SOS), the plan build time, the other
+configurations' legs, the parity of the timed passes' grid against the
+oracle (FP32 restatement of the reference CPU loop and FP64), and the CPU
+baseline: that FP32 restatement timed on this host over the whole workload
+with min(coils, host threads) threads.
 """
 import argparse
 import json
@@ -38,7 +40,6 @@ def parse():
     ap.add_argument("--readout", type=int, default=256)
     ap.add_argument("--coils", type=int, default=32)
     ap.add_argument("--image", type=int, default=128)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline work (seconds)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recon", action="store_true")
     ap.add_argument("--no-ldcf", action="store_true", help="skip the learned-vs-iterative DCF leg")
@@ -53,7 +54,7 @@ def parse():
     ap.add_argument("--xcd-run", type=int, default=0, help="accumulate 6: items per XCD run (0: automatic)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the HBM-counter calibration kernel) and exit")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "grid_kooshball3d_c32_traffic.txt"),
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "grid_pos_kooshball3d_c32_traffic.txt"),
                     help="counter summary giving the gridding pass's HBM bytes (FETCH_SIZE + WRITE_SIZE)")
     return ap.parse_args()
 
@@ -77,10 +78,26 @@ GRID_KERNELS = {1: "k_grid_items (LDS atomic add)", 2: "k_grid_items (LDS read-a
                 6: "k_grid_pos (matrix cores, weights from sample positions)"}
 
 
+# sources whose code decides the gridding pass's memory traffic; a counter
+# summary applies only to the build it was measured on
+TRAFFIC_SOURCES = ("fastrecon_amd/csrc/fr_grid_pos.hip", "fastrecon_amd/csrc/fr_plan_pos.hip",
+                   "fastrecon_amd/csrc/fr_grid.hip", "fastrecon_amd/csrc/fr_grid_mfma.hip",
+                   "fastrecon_amd/csrc/fr_plan.hip")
+
+
+def sources_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(path, workload, kernel):
     """HBM bytes per gridding pass from a committed counter summary, used
-    only when it was measured on this exact workload and kernel (kernel name
-    and unit box)."""
+    only when it was measured on this exact workload, kernel and build (the
+    hash of the gridding sources recorded in the summary)."""
     try:
         kv = {}
         with open(path) as f:
@@ -88,7 +105,7 @@ def read_traffic(path, workload, kernel):
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
                     kv[k.strip()] = v.strip()
-        if kv.get("workload") != workload or kv.get("kernel") != kernel:
+        if kv.get("workload") != workload or kv.get("kernel") != kernel or kv.get("sources_sha16") != sources_hash():
             return None, None
         return float(kv["traffic_bytes_per_pass"]), os.path.relpath(path, ROOT)
     except (OSError, KeyError, ValueError):
@@ -239,42 +256,69 @@ def bench_radial2d(ctx, torch, hip, syn, steps):
     return res
 
 
-def cpu_baseline(traj_n, dcf, N, coils, lut, target_s):
-    """Oracle FP32 restatement of GridLut::griddingChannel on a bounded
-    slice (the first spokes of the same trajectory), OpenMP over coils with
-    min(coils, host threads) threads as Program/Main.cpp:111."""
+def cpu_info():
+    """Host threads available to this process, all online threads, CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return avail, os.cpu_count() or avail, model
+
+
+def cpu_baseline_and_parity(torch, hip, traj_n, dcf, N, coils, lut, grid_dev, seed):
+    """The oracle's FP32 restatement of GridLut::griddingChannel
+    (GridLut.cpp:80-147) on the whole workload, OpenMP over coils with
+    min(coils, host threads) threads as Program/Main.cpp:111 sets them,
+    timed; then the device grid's relative L2 error against it and against
+    the FP64 oracle (coil by coil), on the same coil data."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(coils, cores, 16))
-    readout = 256
-    m = readout * 64
-    rng = np.random.default_rng(12345)
-
-    def data(n):
-        return (rng.standard_normal((coils, n), np.float32)
-                + 1j * rng.standard_normal((coils, n), np.float32)).astype(np.complex64)
-
-    # grow the slice until one timed run takes at least target_s (or the
-    # whole trajectory is used)
+    import concurrent.futures as cf
+    avail, online, model = cpu_info()
+    threads = max(1, min(coils, avail))
     M = traj_n.shape[0]
-    while True:
-        tr = traj_n[:m]
-        d = data(m)
-        t0 = time.perf_counter()
-        oracle.grid_f32(tr, dcf[:m], d, N, 4.0, lut, nthreads=threads)
-        dt = time.perf_counter() - t0
-        if dt >= target_s or m >= M:
-            break
-        m = min(M, max(2 * m, int(m * target_s / max(dt, 1e-3) * 1.1)) // readout * readout)
-    m2 = m
-    return {"value": coils * m2 / dt, "unit": "coil-samples/s", "cores": threads, "kind": "port",
-            "sample": f"first {m2 // readout} spokes x {readout} samples x {coils} coils of the same workload "
-                      f"on the full {N}^3 grid, {dt:.2f} s, oracle FP32 restatement of GridLut.cpp:80-147 "
-                      f"(OpenMP over coils)"}
+    # the device's coil data (same seed), to the host, coil-major
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    d = torch.randn(M * coils * 2, generator=gen, device="cuda", dtype=torch.float32)
+    data = np.ascontiguousarray(d.view(M, coils, 2).cpu().numpy().view(np.complex64).reshape(M, coils).T)
+    del d
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    ref = oracle.grid_f32(traj_n, dcf, data, N, 4.0, lut, nthreads=threads)
+    dt = time.perf_counter() - t0
+    got = grid_dev.download_all()
+    num = den = num64 = num_cpu = 0.0
+    for c in range(coils):
+        num += float(np.sum(np.abs(got[c] - ref[c]).astype(np.float64) ** 2))
+        den += float(np.sum(np.abs(ref[c]).astype(np.float64) ** 2))
+
+    def f64(c):
+        return c, oracle.grid_f64(traj_n, dcf, data[c:c + 1], N, 4.0, lut, nthreads=1)[0]
+    den64 = 0.0
+    with cf.ThreadPoolExecutor(threads) as ex:
+        for c, r64 in ex.map(f64, range(coils)):
+            num64 += float(np.sum(np.abs(got[c].astype(np.complex128) - r64) ** 2))
+            num_cpu += float(np.sum(np.abs(ref[c].astype(np.complex128) - r64) ** 2))
+            den64 += float(np.sum(np.abs(r64) ** 2))
+    base = {"value": coils * M / dt, "unit": "coil-samples/s", "cores": threads, "kind": "port",
+            "host_threads_available": avail, "host_threads_online": online, "cpu_model": model,
+            "sample": f"the whole workload ({M} samples x {coils} coils on the {N}^3 grid), {dt:.2f} s, "
+                      f"oracle FP32 restatement of GridLut.cpp:80-147, OpenMP over coils with min(coils, "
+                      f"host threads) threads (Main.cpp:111)"}
+    parity = {"rel_l2_vs_cpu_fp32": float(np.sqrt(num / den)), "rel_l2_vs_fp64": float(np.sqrt(num64 / den64)),
+              "cpu_fp32_vs_fp64": float(np.sqrt(num_cpu / den64)), "bar": "rel-L2 <= 1e-5 vs the CPU FP32 path",
+              "data": "the timed passes' coil data, every sample and coil"}
+    return base, parity
 
 
 def main():
@@ -366,7 +410,8 @@ def main():
                      "traffic_source": traffic_src,
                      "algorithmic_bytes": bytes_alg,
                      "kernel": f"gridding pass: {kernel} + k_combine (fr_grid_execute), HIP events on the "
-                               "context stream"},
+                               "context stream",
+                     "sources_sha16": sources_hash()},
         "plan_ms": plan_ms, "plan_taps": info["taps"], "plan_bytes": info["plan_bytes"],
     }
 
@@ -441,7 +486,20 @@ def main():
         mlp.close()
 
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(traj_n, dcf, N, C, lut, a.cpu_seconds)
+        # the timed passes' grid, recomputed on the same data (the legs above
+        # reused the device memory)
+        s2 = hip.Samples(ctx, 3, M, C)
+        s2.set_traj(traj_n)
+        s2.set_dcf(dcf)
+        device_data(torch, ctx, s2, M, C, 2)
+        plan2 = hip.Plan(ctx, s2, N, kw, lut, tune)
+        g2 = hip.Grids(ctx, 3, N, C)
+        hip.grid_execute(plan2, s2, g2)
+        ctx.sync()
+        plan2.close()
+        s2.close()
+        out["cpu_baseline"], out["parity"] = cpu_baseline_and_parity(torch, hip, traj_n, dcf, N, C, lut, g2, 2)
+        g2.close()
     print(json.dumps(out), flush=True)
 
 
