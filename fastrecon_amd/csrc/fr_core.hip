@@ -45,6 +45,12 @@ extern "C" fr_status fr_ctx_create(int device, fr_ctx **out)
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return FR_ERR_HIP; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    // this device's static scratch for fr_grid_execute_noalloc
+    if (fr_scratch_init(c) != FR_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return FR_ERR_HIP;
+    }
     *out = c;
     return FR_OK;
 }

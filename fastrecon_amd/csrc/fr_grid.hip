@@ -434,7 +434,7 @@ __global__ void k_combine(const fr_heavy *__restrict__ heavy, const fr_unit *__r
 }
 
 template <int ACC, typename TS>
-fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
     fr_ctx *ctx = p->ctx;
     const fr_geom &g = p->geom;
@@ -451,13 +451,13 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
         const size_t glds = sizeof(float) * (size_t)(GB + 1) * 64;
         hipLaunchKernelGGL(k_grid_gather<TS>, grid, dim3(64), glds, ctx->stream, p->items, p->nitems, p->units,
                            p->batches, p->hdrs, p->taps, (const TS *)s->data, s->dcf, s->C, (int)p->N, cells, out,
-                           (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
+                           partial, p->nslots, g.cells_max, p->nitems_front);
     } else {
         auto kern = k_grid_items<ACC == FR_ACC_LDS_ATOMIC ? 1 : 0, TS>;
         FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(kern, grid, dim3(64), lds, ctx->stream, p->items, p->nitems, p->units, p->hdrs, p->taps,
                            (const TS *)s->data, s->dcf, s->C, stride, acc_floats, (int)p->N, cells, out,
-                           (float2 *)p->partial, p->nslots, g.cells_max, p->nitems_front);
+                           partial, p->nslots, g.cells_max, p->nitems_front);
     }
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
@@ -465,14 +465,28 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
 
 }  // namespace
 
-fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode)
+size_t fr_grid_scratch_bytes(const fr_plan *p, int C)
+{
+    const int chunks = (C + 31) / 32;
+    return (size_t)chunks * p->nslots * 32 * p->geom.cells_max * sizeof(float2);
+}
+
+fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode, float2 *scratch,
+                         size_t scratch_bytes)
 {
     fr_ctx *ctx = p->ctx;
     const fr_geom &g = p->geom;
-    // heavy-unit slices: partial boxes [32-coil chunk][slot][32 coils][cells_max]
+    // heavy-unit slices: partial boxes [32-coil chunk][slot][32 coils][cells_max],
+    // in the caller's scratch or in the plan's own buffer (grown on demand)
     const int chunks = (s->C + 31) / 32;
-    if (p->nslots > 0) {
-        const size_t need = (size_t)chunks * p->nslots * 32 * g.cells_max * sizeof(float2);
+    const size_t need = fr_grid_scratch_bytes(p, s->C);
+    float2 *partial = nullptr;
+    if (scratch) {
+        FR_CHECK(ctx, need <= scratch_bytes, FR_ERR_UNSUPPORTED,
+                 "this plan's heavy-unit slices need %zu bytes of scratch for %d coils; the scratch holds %zu",
+                 need, s->C, scratch_bytes);
+        partial = scratch;
+    } else {
         if (need > p->partial_bytes) {
             FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
             hipFree(p->partial);
@@ -482,40 +496,42 @@ fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force
             if (e) return e;
             p->partial_bytes = need;
         }
+        partial = (float2 *)p->partial;
     }
     if (p->nitems == 0) return FR_OK;
     const bool c64 = s->type == FR_SAMPLES_C64;
     fr_status e;
     switch (force_mode ? force_mode : p->accumulate) {
     case FR_ACC_MFMA:
-        e = fr_launch_grid_mfma(p, s, out);
+        e = fr_launch_grid_mfma(p, s, out, partial);
         break;
     case FR_ACC_POS:
-        e = fr_launch_grid_pos(p, s, out);
+        e = fr_launch_grid_pos(p, s, out, partial);
         break;
     case FR_ACC_GATHER:
-        e = c64 ? launch<FR_ACC_GATHER, float2>(p, s, out) : launch<FR_ACC_GATHER, __half2>(p, s, out);
+        e = c64 ? launch<FR_ACC_GATHER, float2>(p, s, out, partial) : launch<FR_ACC_GATHER, __half2>(p, s, out, partial);
         break;
     case FR_ACC_LDS_ATOMIC:
-        e = c64 ? launch<FR_ACC_LDS_ATOMIC, float2>(p, s, out) : launch<FR_ACC_LDS_ATOMIC, __half2>(p, s, out);
+        e = c64 ? launch<FR_ACC_LDS_ATOMIC, float2>(p, s, out, partial)
+                : launch<FR_ACC_LDS_ATOMIC, __half2>(p, s, out, partial);
         break;
     default:
-        e = c64 ? launch<FR_ACC_LDS_RMW, float2>(p, s, out) : launch<FR_ACC_LDS_RMW, __half2>(p, s, out);
+        e = c64 ? launch<FR_ACC_LDS_RMW, float2>(p, s, out, partial) : launch<FR_ACC_LDS_RMW, __half2>(p, s, out, partial);
         break;
     }
     if (e) return e;
     if (p->nheavy > 0) {
         const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
         hipLaunchKernelGGL(k_combine, dim3((unsigned)p->nheavy, chunks), dim3(256), 0, ctx->stream, p->heavy,
-                           p->units, (const float2 *)p->partial, p->nslots, g.cells_max, s->C, (int)p->N, cells, out);
+                           p->units, (const float2 *)partial, p->nslots, g.cells_max, s->C, (int)p->N, cells, out);
         FR_HIP(ctx, hipGetLastError());
     }
     return FR_OK;
 }
 
-extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *out)
+// Argument checks shared by the gridding entry points.
+fr_status fr_grid_check(const fr_plan *p, const fr_samples *s, const fr_grids *out)
 {
-    if (!p || !s || !out) return FR_ERR_INVALID;
     fr_ctx *ctx = p->ctx;
     FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID,
              "samples (dim %d, M %lld) do not match the plan (dim %d, M %lld)", s->dim, (long long)s->M, p->dim,
@@ -527,5 +543,13 @@ extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *
     FR_CHECK(ctx, s->frames == p->frames, FR_ERR_INVALID, "samples and plan differ in frames");
     FR_CHECK(ctx, s->has_dcf, FR_ERR_STATE, "DCF not set");
     FR_CHECK(ctx, p->geom.cells_max <= 64, FR_ERR_INVALID, "plan unit box exceeds 64 cells");
+    return FR_OK;
+}
+
+extern "C" fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *out)
+{
+    if (!p || !s || !out) return FR_ERR_INVALID;
+    fr_status e = fr_grid_check(p, s, out);
+    if (e) return e;
     return fr_launch_grid(p, s, out->data, 0);
 }

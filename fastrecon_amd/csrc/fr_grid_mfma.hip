@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(64) k_grid_mfma(const fr_item *__restrict__ it
 }
 
 template <int NBLK, int NCB, bool PAIR, bool CT, bool FULL, typename TS>
-fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
     fr_ctx *ctx = p->ctx;
     // the kernel addresses headers, tap starts, compact taps and DCFs by 32-bit byte offsets
@@ -410,42 +410,43 @@ fr_status launch_mfma_ct(fr_plan *p, const fr_samples *s, float2 *out)
     hipLaunchKernelGGL((k_grid_mfma<NBLK, NCB, PAIR, CT, FULL, TS>), dim3((unsigned)W, chunks), dim3(64), 0, ctx->stream,
                        p->items, p->nitems, p->units, p->hdrs, p->tstart, (const void *)p->taps, (const TS *)s->data,
                        s->dcf,
-                       s->C, (int)p->N, cells, out, (float2 *)p->partial, p->nslots, p->geom.cells_max, p->lut,
+                       s->C, (int)p->N, cells, out, partial, p->nslots, p->geom.cells_max, p->lut,
                        p->L, p->nitems_front, (uint32_t)p->ntaps);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
 
 template <int NBLK, int NCB, bool PAIR, bool FULL, typename TS>
-fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch_mfma(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
-    return p->compact_taps ? launch_mfma_ct<NBLK, NCB, PAIR, true, FULL, TS>(p, s, out)
-                           : launch_mfma_ct<NBLK, NCB, PAIR, false, FULL, TS>(p, s, out);
+    return p->compact_taps ? launch_mfma_ct<NBLK, NCB, PAIR, true, FULL, TS>(p, s, out, partial)
+                           : launch_mfma_ct<NBLK, NCB, PAIR, false, FULL, TS>(p, s, out, partial);
 }
 
 // FULL: every chunk holds 8 NCB coils, so no lane's coil is masked
 template <int NBLK, typename TS>
-fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
-    if (s->C <= 8) return launch_mfma<NBLK, 1, false, false, TS>(p, s, out);
-    if (s->C <= 16) return launch_mfma<NBLK, 2, false, false, TS>(p, s, out);
-    if (s->C % 32 == 0) return launch_mfma<NBLK, 4, true, true, TS>(p, s, out);
-    if (s->C % 2 == 0) return launch_mfma<NBLK, 4, true, false, TS>(p, s, out);
-    return launch_mfma<NBLK, 4, false, false, TS>(p, s, out);
+    if (s->C <= 8) return launch_mfma<NBLK, 1, false, false, TS>(p, s, out, partial);
+    if (s->C <= 16) return launch_mfma<NBLK, 2, false, false, TS>(p, s, out, partial);
+    if (s->C % 32 == 0) return launch_mfma<NBLK, 4, true, true, TS>(p, s, out, partial);
+    if (s->C % 2 == 0) return launch_mfma<NBLK, 4, true, false, TS>(p, s, out, partial);
+    return launch_mfma<NBLK, 4, false, false, TS>(p, s, out, partial);
 }
 
 template <typename TS>
-fr_status launch_nblk(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch_nblk(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
     const int cm = p->geom.cells_max;
-    if (cm <= 16) return launch_ncb<1, TS>(p, s, out);
-    if (cm <= 32) return launch_ncb<2, TS>(p, s, out);
-    return launch_ncb<4, TS>(p, s, out);
+    if (cm <= 16) return launch_ncb<1, TS>(p, s, out, partial);
+    if (cm <= 32) return launch_ncb<2, TS>(p, s, out, partial);
+    return launch_ncb<4, TS>(p, s, out, partial);
 }
 
 }  // namespace
 
-fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
-    return s->type == FR_SAMPLES_C64 ? launch_nblk<float2>(p, s, out) : launch_nblk<__half2>(p, s, out);
+    return s->type == FR_SAMPLES_C64 ? launch_nblk<float2>(p, s, out, partial)
+                                     : launch_nblk<__half2>(p, s, out, partial);
 }

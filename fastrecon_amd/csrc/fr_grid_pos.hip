@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
 }
 
 template <int NCB, bool PAIR, bool FULL, typename TS, int OCC>
-fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
     fr_ctx *ctx = p->ctx;
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
@@ -474,33 +474,34 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out)
               lbz = __builtin_ctz((unsigned)p->geom.bz);
     hipLaunchKernelGGL(kern, dim3((unsigned)W, chunks), dim3(64 * PW), lds, ctx->stream, p->items, p->nitems,
                        p->tiles, p->ntiles, lbx, lby, lbz, p->recs, p->pos, p->tsel, p->L, p->mfma_dacc, p->mfma_cidx,
-                       (const TS *)s->data, s->dcf, s->C, (int)p->N, cells, out, (float2 *)p->partial, p->nslots,
+                       (const TS *)s->data, s->dcf, s->C, (int)p->N, cells, out, partial, p->nslots,
                        p->geom.cells_max, p->nitems_front, front_wg, ipw, xr, p->pos_stores == 1 ? 1 : 0);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
 
 template <typename TS, int OCC>
-fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
-    if (s->C <= 8) return launch<1, false, false, TS, OCC>(p, s, out);
-    if (s->C <= 16) return launch<2, false, false, TS, OCC>(p, s, out);
-    if (s->C % 32 == 0) return launch<4, true, true, TS, OCC>(p, s, out);
-    if (s->C % 2 == 0) return launch<4, true, false, TS, OCC>(p, s, out);
-    return launch<4, false, false, TS, OCC>(p, s, out);
+    if (s->C <= 8) return launch<1, false, false, TS, OCC>(p, s, out, partial);
+    if (s->C <= 16) return launch<2, false, false, TS, OCC>(p, s, out, partial);
+    if (s->C % 32 == 0) return launch<4, true, true, TS, OCC>(p, s, out, partial);
+    if (s->C % 2 == 0) return launch<4, true, false, TS, OCC>(p, s, out, partial);
+    return launch<4, false, false, TS, OCC>(p, s, out, partial);
 }
 
 template <typename TS>
-fr_status launch_occ(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status launch_occ(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
-    return p->pos_waves == 4 ? launch_ncb<TS, 4>(p, s, out) : launch_ncb<TS, 1>(p, s, out);
+    return p->pos_waves == 4 ? launch_ncb<TS, 4>(p, s, out, partial) : launch_ncb<TS, 1>(p, s, out, partial);
 }
 
 }  // namespace
 
-fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out)
+fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
     FR_CHECK(p->ctx, p->recs && p->pos && p->tsel && p->geom.cells_max <= 16 && p->L <= FR_POS_MAX_L,
              FR_ERR_STATE, "position-weight gridding needs a plan built for it (accumulate 6)");
-    return s->type == FR_SAMPLES_C64 ? launch_occ<float2>(p, s, out) : launch_occ<__half2>(p, s, out);
+    return s->type == FR_SAMPLES_C64 ? launch_occ<float2>(p, s, out, partial)
+                                     : launch_occ<__half2>(p, s, out, partial);
 }

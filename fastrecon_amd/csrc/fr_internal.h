@@ -17,6 +17,7 @@ struct fr_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     int cu_count = 0;
+    void *scratch = nullptr;   // this device's copy of the static scratch (fr_noalloc.hip)
 };
 
 struct fr_samples {
@@ -237,9 +238,18 @@ static inline size_t fr_plan_cells(int dim, unsigned N, int frames)
 // Launch helpers implemented in the kernel translation units.
 // force_mode: 0 = the plan's accumulation mode, else that mode (any plan's
 // records serve FR_ACC_MFMA when its LUT has at most FR_MFMA_MAX_L entries)
-fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode);
-fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out);
-fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out);
+// scratch: the heavy-unit slice partials go there (of scratch_bytes; no
+// allocation), else into the plan's own buffer, grown on demand
+fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode, float2 *scratch = nullptr,
+                         size_t scratch_bytes = 0);
+fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial);
+fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial);
+// bytes of heavy-unit slice partials a pass of C coils needs
+size_t fr_grid_scratch_bytes(const fr_plan *p, int C);
+fr_status fr_grid_check(const fr_plan *p, const fr_samples *s, const fr_grids *out);
+// the no-allocation entry point's static scratch (fr_noalloc.hip): its
+// address on the context's device, looked up once per device
+fr_status fr_scratch_init(fr_ctx *ctx);
 // cropped FFT + SOS helpers (fr_fftcrop.hip)
 fr_status fr_fft_workspace(fr_fft *f, size_t bytes);
 // FR_ERR_INVALID unless [p, p + bytes) lies inside the device allocation of p

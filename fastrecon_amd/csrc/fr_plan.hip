@@ -966,5 +966,6 @@ extern "C" fr_status fr_plan_get_info(const fr_plan *p, fr_plan_info *info)
     info->max_unit_taps = p->max_unit_taps;
     info->exact_order = p->exact_order;
     info->accumulate = p->accumulate;
+    info->scratch_bytes = (int64_t)fr_grid_scratch_bytes(p, 32);
     return FR_OK;
 }

@@ -31,7 +31,7 @@ EXPORTS = [
     "fr_samples_set_traj_device", "fr_samples_get_dcf",
     "fr_plan_create", "fr_plan_destroy", "fr_plan_get_info",
     "fr_grids_create", "fr_grids_create_frames", "fr_samples_create_frames", "fr_grids_destroy", "fr_grids_download", "fr_grids_upload", "fr_grids_device_ptr",
-    "fr_grid_execute", "fr_degrid_execute",
+    "fr_grid_execute", "fr_grid_execute_noalloc", "fr_degrid_execute",
     "fr_fft_create", "fr_fft_destroy", "fr_fft_execute",
     "fr_fftshift", "fr_sos", "fr_fft_sos", "fr_deapod_map", "fr_lowfilter", "fr_normalize", "fr_crop", "fr_sos_map",
     "fr_grids_copy",
@@ -59,7 +59,7 @@ class fr_plan_info(C.Structure):
                 ("tile_x", C.c_int), ("tile_y", C.c_int), ("tile_z", C.c_int), ("slabs", C.c_int),
                 ("tiles", C.c_int), ("lds_bytes", C.c_int), ("plan_bytes", C.c_int64), ("build_ms", C.c_double),
                 ("items", C.c_int64), ("heavy_units", C.c_int64), ("max_unit_taps", C.c_int64),
-                ("exact_order", C.c_int), ("accumulate", C.c_int)]
+                ("exact_order", C.c_int), ("accumulate", C.c_int), ("scratch_bytes", C.c_int64)]
 
 
 _lib = None
@@ -106,6 +106,7 @@ def load():
         "fr_grids_download": [_vp, C.c_int, _vp],
         "fr_grids_upload": [_vp, C.c_int, _vp],
         "fr_grid_execute": [_vp, _vp, _vp],
+        "fr_grid_execute_noalloc": [_vp, _vp, _vp],
         "fr_degrid_execute": [_vp, _vp, _vp, _vp],
         "fr_fft_create": [_vp, C.c_int, C.c_uint, C.c_int, C.c_int, _pp],
         "fr_fft_destroy": [_vp],
@@ -424,6 +425,12 @@ class DevicePtr:
 # ----------------------------------------------------------------- calls
 def grid_execute(plan, samples, grids):
     plan.ctx.check(plan.ctx.L.fr_grid_execute(plan.h, samples.h, grids.h), "fr_grid_execute")
+
+
+def grid_execute_noalloc(plan, samples, grids):
+    """fr_grid_execute_noalloc: the same pass with no device allocation (the
+    slice partials in the library's static per-GPU scratch)."""
+    plan.ctx.check(plan.ctx.L.fr_grid_execute_noalloc(plan.h, samples.h, grids.h), "fr_grid_execute_noalloc")
 
 
 def degrid_execute(plan, samples, grids, out_buf):

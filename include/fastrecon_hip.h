@@ -109,6 +109,7 @@ typedef struct {
     int64_t items, heavy_units, max_unit_taps;
     int exact_order;
     int accumulate;              /* the plan's resolved accumulation mode (1, 2, 4, 5 or 6) */
+    int64_t scratch_bytes;       /* heavy-unit slice partials of one pass, per 32 coils */
 } fr_plan_info;
 
 /* ------------------------------------------------------------ host helpers */
@@ -195,6 +196,17 @@ void *fr_grids_device_ptr(fr_grids *g);
  * (cuGridLut.cu:28-48, 231-309) and GridLut::execute (GridLut.cpp:53-78):
  * grids every coil of s into out (all coils in one launch). */
 fr_status fr_grid_execute(fr_plan *p, const fr_samples *s, fr_grids *out);
+
+/* fr_grid_execute without any device allocation: the heavy-unit slices'
+ * partial sums, the only device memory a pass needs beyond its inputs and
+ * outputs (which fr_grid_execute allocates in the plan on first use), go to
+ * a static 32 MiB scratch array that each GPU holds for the life of the
+ * process.  No counterpart in the reference; for callers that cannot
+ * allocate (HIP graph capture, serving loops).  Same kernels and results as
+ * fr_grid_execute.  Fails with FR_ERR_UNSUPPORTED when the plan needs more
+ * scratch than the array holds (fr_plan_info.scratch_bytes per 32 coils).
+ * Calls that overlap on one GPU share that array and race. */
+fr_status fr_grid_execute_noalloc(fr_plan *p, const fr_samples *s, fr_grids *out);
 
 /* Degridding (grid -> samples), the adjoint of fr_grid_execute with the
  * same taps, no DCF.  Net-new (the reference has none).  s supplies the
