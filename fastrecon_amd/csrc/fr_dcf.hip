@@ -105,6 +105,57 @@ __global__ void k_pm_update(float *__restrict__ w, const float2 *__restrict__ de
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Per-block FP64 partial sums of |g|^2 over n grid cells and of w over M
+// samples: part[2b] and part[2b+1].  Fixed grid-stride order and a fixed
+// reduction tree, so the result does not vary between runs.
+constexpr int SUM_BLOCKS = 1024;
+__global__ void __launch_bounds__(256) k_identity_partials(const float2 *__restrict__ g, int64_t n,
+                                                           const float *__restrict__ w, int64_t M,
+                                                           double *__restrict__ part)
+{
+    double a = 0.0, b = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float2 v = g[i];
+        a += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride) b += (double)w[i];
+    for (int m = 32; m >= 1; m >>= 1) {
+        a += __shfl_xor(a, m);
+        b += __shfl_xor(b, m);
+    }
+    __shared__ double sa[4], sb[4];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sa[wv] = a;
+        sb[wv] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = (sa[0] + sa[1]) + (sa[2] + sa[3]);
+        part[2 * blockIdx.x + 1] = (sb[0] + sb[1]) + (sb[2] + sb[3]);
+    }
+}
+
+// alpha = sum(w) / sum |G w|^2 from the partials (one thread, in order),
+// then w <- alpha w by every thread of the grid
+__global__ void k_identity_alpha(const double *__restrict__ part, int nb, float *__restrict__ alpha)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    alpha[0] = a > 0.0 ? (float)(b / a) : 0.f;
+}
+
+__global__ void k_scale_by(float *__restrict__ w, int64_t M, const float *__restrict__ alpha)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) w[i] = __fmul_rn(w[i], alpha[0]);
+}
+
 }  // namespace
 
 fr_status fr_degrid_raw(fr_plan *p, const fr_samples *s, const float2 *grid, int C, float2 *out)
@@ -190,6 +241,55 @@ extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int
         }
     }
     hipFree(den);
+    fr_grids_destroy(g);
+    fr_samples_destroy(u);
+    return st;
+}
+
+// Global scale of a DCF from the Pipe-Menon fixed-point identity: at
+// convergence G^H G w = 1 on every sample, so sum(w) = <w, G^H G w> =
+// |G w|^2.  alpha = sum(w) / |G w|^2 restores that for any w, and is exact
+// for a DCF that differs from the fixed point by a constant factor.  One
+// single-channel gridding pass of w on this plan, two reductions, no host
+// round trip; w is scaled in place.
+extern "C" fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, float *alpha_out)
+{
+    if (!p || !s) return FR_ERR_INVALID;
+    fr_ctx *ctx = p->ctx;
+    FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID, "samples do not match the plan");
+    FR_CHECK(ctx, s->frames == 0 && p->frames == 0, FR_ERR_UNSUPPORTED,
+             "identity scale of batched frames: scale each frame on its own plan");
+    FR_CHECK(ctx, s->has_traj && s->has_dcf, FR_ERR_STATE, "samples need a trajectory and a DCF");
+    const int64_t M = s->M;
+    fr_samples *u = nullptr;
+    fr_grids *g = nullptr;
+    double *part = nullptr;
+    float *alpha = nullptr;
+    fr_status st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &u);
+    if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &g);
+    if (!st) st = fr_alloc(ctx, (void **)&part, sizeof(double) * 2 * SUM_BLOCKS + sizeof(float));
+    if (!st) {
+        alpha = (float *)(part + 2 * SUM_BLOCKS);
+        hipMemcpyAsync(u->traj, s->traj, sizeof(float) * s->dim * M, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+        hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, (float2 *)u->data, M);
+        u->has_traj = u->has_dcf = true;
+        const int mode = p->accumulate == FR_ACC_POS ? FR_ACC_POS
+                         : (p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
+        st = fr_launch_grid(p, u, g->data, mode);
+    }
+    if (!st) {
+        hipLaunchKernelGGL(k_identity_partials, dim3(SUM_BLOCKS), dim3(256), 0, ctx->stream, g->data,
+                           (int64_t)g->cells, s->dcf, M, part);
+        hipLaunchKernelGGL(k_identity_alpha, dim3(1), dim3(64), 0, ctx->stream, part, SUM_BLOCKS, alpha);
+        hipLaunchKernelGGL(k_scale_by, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, s->dcf, M, alpha);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && alpha_out)
+            e = hipMemcpyAsync(alpha_out, alpha, sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = fr_fail(ctx, FR_ERR_HIP, "identity scale: %s", hipGetErrorString(e));
+    }
+    hipFree(part);
     fr_grids_destroy(g);
     fr_samples_destroy(u);
     return st;

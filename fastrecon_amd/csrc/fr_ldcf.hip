@@ -1,17 +1,24 @@
 // Learned density compensation (net-new; the reference has no DCF
-// estimation): per-sample trajectory features and a 6-64-64-1 SiLU MLP with
+// estimation): per-sample trajectory features and an 8-64-64-1 SiLU MLP with
 // BF16 weight matrices and FP32 accumulation, trained and applied on the
 // device.
 //
-// Features (one row of 8 floats per sample, the last two zero):
-//   u = (t - (N-1)/2) / (N/2) for the normalised point t,
-//   f0 = log1p(|u| N/2)/5 (the radius |k| in grid cells, log-compressed so
-//   that the first cells around the k-space centre are resolved),
+// Features (one row of 8 floats per sample):
+//   u = (t - (N-1)/2) / (N/2) for the normalised point t, rho = |u| N/2
+//   (the radius |k| in grid cells),
+//   f0 = log1p(rho)/5 (log-compressed so that the first cells around the
+//   k-space centre are resolved),
 //   f1 = atan2(u_y, u_x)/pi, f2 = acos(u_z/|u|)/pi (0.5 in 2D),
 //   f3..f5 = log1p(n_r)/8 with n_r the number of other samples whose nearest
-//   cell lies within Chebyshev distance r = 1, 2, 4 of this sample's cell.
+//   cell lies within Chebyshev distance r = 1, 2, 4 of this sample's cell,
+//   f6 = min(rho_max - rho, 4)/4, the distance in cells to the outermost
+//   sample's shell (rho_max = the largest rho of all samples): the outermost
+//   shells (a spoke's end samples) carry the largest weights and differ by
+//   one cell, which f0 resolves only to 0.002,
+//   f7 = log1p(n_0)/8, the other samples in this sample's own cell.
 // The counts come from a cell histogram and its summed-area table, so each
-// sample costs 3 box sums (8 corner reads each in 3D) whatever the density.
+// sample costs 4 box sums (8 corner reads each in 3D) whatever the density;
+// rho_max is one atomic maximum per sample in the histogram pass.
 //
 // MLP layer 2 (64 x 64) runs on MFMA (v_mfma_f32_16x16x32_bf16): a wave
 // takes 16 samples, computes h1 = silu(x W1 + b1) for them in VGPRs already
@@ -31,7 +38,7 @@
 
 namespace {
 
-constexpr int NF = 6;
+constexpr int NF = 8;
 constexpr int H = 64;
 constexpr int FSTRIDE = 8;   // floats per feature row
 constexpr int O_W1 = 0, O_B1 = NF * H, O_W2 = O_B1 + H, O_B2 = O_W2 + H * H, O_W3 = O_B2 + H, O_B3 = O_W3 + H;
@@ -77,6 +84,20 @@ __device__ inline float dsilu(float z)
 constexpr int HR = 3;                        // central box half-width (cells)
 constexpr int HB = 2 * HR + 1;
 
+// FP32 radius in cells of the normalised point (same operations as
+// sample_features)
+__device__ inline float sample_rho(const float *__restrict__ traj, int64_t M, int dim, int N, int64_t s)
+{
+    const float cf = 0.5f * (float)(N - 1), hN = 0.5f * (float)N;
+    float u[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < dim; ++d) u[d] = __fdiv_rn(__fsub_rn(traj[d * M + s], cf), hN);
+    const float r2 = dim == 3 ? __fadd_rn(__fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1])), __fmul_rn(u[2], u[2]))
+                              : __fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1]));
+    return __fmul_rn(sqrtf(r2), hN);
+}
+
+// hist[cells] (one past the histogram) receives rho_max as float bits
+// (non-negative floats order like their bits)
 __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ traj, int64_t M, int dim, int N,
                                                    uint32_t *__restrict__ hist)
 {
@@ -86,7 +107,10 @@ __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ tra
     __syncthreads();
     const int lo = N / 2 - HR;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
+    float rmax = 0.f;
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < M; s += stride) {
+        rmax = fmaxf(rmax, sample_rho(traj, M, dim, N, s));
         size_t cell = 0;
         int hidx = 0;
         bool central = true;
@@ -103,6 +127,9 @@ __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ tra
         else
             atomicAdd(&hist[cell], 1u);
     }
+    // wave maximum, then one atomic per wave
+    for (int m = 32; m >= 1; m >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, m));
+    if ((threadIdx.x & 63) == 0) atomicMax(&hist[ncells], __float_as_uint(rmax));
     __syncthreads();
     for (int i = threadIdx.x; i < nhot; i += blockDim.x) {
         const uint32_t v = hot[i];
@@ -180,7 +207,8 @@ __device__ inline uint32_t box_count(const uint32_t *__restrict__ S, int dim, in
            sat_at(S, N, hi[0], lo[1], lo[2]) - sat_at(S, N, lo[0], lo[1], lo[2]);
 }
 
-// the six features of sample s (f[6], f[7] = 0)
+// the eight features of sample s; S: summed-area table of the cell counts,
+// followed by rho_max (float bits)
 __device__ inline void sample_features(const float *__restrict__ traj, int64_t M, int dim, int N,
                                        const uint32_t *__restrict__ S, int64_t s, float (&f)[FSTRIDE])
 {
@@ -201,8 +229,11 @@ __device__ inline void sample_features(const float *__restrict__ traj, int64_t M
     f[2] = (dim == 3 && r > 0.f) ? acosf(fminf(fmaxf(u[2] / r, -1.f), 1.f)) * inv_pi : 0.5f;
     const int rr[3] = {1, 2, 4};
     for (int q = 0; q < 3; ++q) f[3 + q] = log1pf((float)(box_count(S, dim, N, c, rr[q]) - 1u)) * 0.125f;
-    f[6] = 0.f;
-    f[7] = 0.f;
+    const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
+    const float rho_max = __uint_as_float(S[ncells]);
+    const float rho = __fmul_rn(r, hN);
+    f[6] = fminf(__fsub_rn(rho_max, rho), 4.f) * 0.25f;
+    f[7] = log1pf((float)(box_count(S, dim, N, c, 0) - 1u)) * 0.125f;
 }
 
 __global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, const uint32_t *__restrict__ S,
@@ -330,11 +361,90 @@ __device__ inline void mlp_forward16(const WS &W, const u32x4 (&a)[4][2], const 
     F.y = __fadd_rn(p2, W.b3());
 }
 
+// ------------------------------------------------------------- inference
+// Inference runs both dense layers on MFMA.  Layer 1 (FP32 16x16x4, two K
+// steps of four features): A = W1^T fragments (lane l: W1[4s + l/16][16mt +
+// l%16]), B = the features (lane l: feature 4s + l/16 of sample l%16), C =
+// b1; the D fragments leave lane l holding z1 of hidden units 16mt +
+// 4(l/16) + r of sample l%16, r = 0..3.  Layer 2 contracts over the hidden
+// units in the order that puts exactly those sixteen units into lane l's
+// BF16 B operand: K position 32ks + 8(l/16) + j holds unit hidden_of(...)
+// = 16(2ks + j/4) + 4(l/16) + j%4, and W2's A fragments are loaded in the
+// same order.  Against the training forward (mlp_forward16, sequential FMAs
+// in layer 1) only the FP32 summation order of z1 differs.
+__device__ inline int hidden_of(int k)
+{
+    const int ks = k >> 5, g = (k >> 3) & 3, j = k & 7;
+    return 16 * (2 * ks + (j >> 2)) + 4 * g + (j & 3);
+}
+
+struct infer_frags {
+    u32x4 a2[4][2];   // W2^T, K positions in hidden_of order
+    float a1[4][2];   // W1^T
+};
+
+__device__ inline void load_infer_frags(const mlp_dev &W, int lane, infer_frags &F)
+{
+    const int m0 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            uint32_t w[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = W.w2[hidden_of(32 * ks + 8 * g + j) * H + 16 * mt + m0];
+            F.a2[mt][ks] = u32x4{w[0] | (w[1] << 16), w[2] | (w[3] << 16), w[4] | (w[5] << 16), w[6] | (w[7] << 16)};
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) F.a1[mt][s] = bf2f(W.w1[(4 * s + g) * H + 16 * mt + m0]);
+    }
+}
+
+// y of the 16 samples of one wave (identical in the four lanes of a
+// sample); x0, x1: features g and 4 + g of sample l%16 (g = l/16)
+template <class WS>
+__device__ inline float mlp_infer16(const WS &W, const infer_frags &F, float x0, float x1, int lane)
+{
+    const int g = lane >> 4;
+    uint32_t hb[4][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const int u = 16 * mt + 4 * g;
+        f32x4 acc = {W.b1(u), W.b1(u + 1), W.b1(u + 2), W.b1(u + 3)};
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.a1[mt][0], x0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.a1[mt][1], x1, acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hb[mt][r] = f2bf(silu(acc[r]));
+    }
+    u32x4 b[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+        b[ks] = u32x4{hb[2 * ks][0] | (hb[2 * ks][1] << 16), hb[2 * ks][2] | (hb[2 * ks][3] << 16),
+                      hb[2 * ks + 1][0] | (hb[2 * ks + 1][1] << 16), hb[2 * ks + 1][2] | (hb[2 * ks + 1][3] << 16)};
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, F.a2[mt][0]),
+                                                     __builtin_bit_cast(bf16x8, b[0]), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, F.a2[mt][1]),
+                                                     __builtin_bit_cast(bf16x8, b[1]), acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int o = 16 * mt + 4 * g + i;
+            part = fmaf(silu(__fadd_rn(acc[i], W.b2(o))), W.w3(o), part);
+        }
+    }
+    const float p1 = __fadd_rn(part, __shfl_xor(part, 16));
+    const float p2 = __fadd_rn(p1, __shfl_xor(p1, 32));
+    return __fadd_rn(p2, W.b3());
+}
+
 __device__ inline void load_x(const float *__restrict__ feat, int64_t row, float *x)
 {
     const float4 *r = reinterpret_cast<const float4 *>(feat + row * FSTRIDE);
     const float4 a = r[0], b = r[1];
-    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y;
+    x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
 }
 
 __device__ inline float mlp_out(float y, int mode, float mu, float sigma)
@@ -348,21 +458,19 @@ __global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__r
 {
     __shared__ lds_weights LW;
     stage_weights(W, LW);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    u32x4 a[4][2];
-    load_a(W, lane, a);
+    infer_frags F;
+    load_infer_frags(W, lane, F);
     __syncthreads();
     const wsrc_lds ws{LW};
     const int64_t groups = (M + 15) / 16;
     for (int64_t gi = wave; gi < groups; gi += nwaves) {
         const int64_t s = gi * 16 + (lane & 15);
-        float x[NF];
-        load_x(feat, s < M ? s : M - 1, x);
-        fwd_state F;
-        mlp_forward16(ws, a, x, lane, F);
-        if (s < M && lane < 16) out[s] = mlp_out(F.y, mode, mu, sigma);
+        const float *row = feat + (s < M ? s : M - 1) * FSTRIDE;
+        const float y = mlp_infer16(ws, F, row[g], row[4 + g], lane);
+        if (s < M && lane < 16) out[s] = mlp_out(y, mode, mu, sigma);
     }
 }
 
@@ -377,11 +485,11 @@ __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__rest
     __shared__ lds_weights LW;
     __shared__ float xs[4][64][NF + 1];
     stage_weights(W, LW);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    u32x4 a[4][2];
-    load_a(W, lane, a);
+    infer_frags F;
+    load_infer_frags(W, lane, F);
     __syncthreads();
     const wsrc_lds ws{LW};
     const int64_t groups = (M + 63) / 64;
@@ -398,13 +506,9 @@ __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__rest
 #pragma unroll 1
         for (int q = 0; q < 4; ++q) {
             const int r = 16 * q + (lane & 15);
-            float x[NF];
-#pragma unroll
-            for (int i = 0; i < NF; ++i) x[i] = xs[wv][r][i];
-            fwd_state F;
-            mlp_forward16(ws, a, x, lane, F);
+            const float y = mlp_infer16(ws, F, xs[wv][r][g], xs[wv][r][4 + g], lane);
             const int64_t s = s0 + r;
-            if (s < M && lane < 16) out[s] = mlp_out(F.y, mode, mu, sigma);
+            if (s < M && lane < 16) out[s] = mlp_out(y, mode, mu, sigma);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -444,7 +548,9 @@ __global__ void __launch_bounds__(256) k_mlp_backward(mlp_dev W, const float *__
     float gW2[H];
 #pragma unroll
     for (int o = 0; o < H; ++o) gW2[o] = 0.f;
-    float gW1[NF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float gW1[NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) gW1[i] = 0.f;
     float gb1 = 0.f, gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, gl = 0.f;
     const float inv2B = 2.0f / (float)B;
     wave_lds &S = L[wv];
@@ -604,10 +710,10 @@ extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned
     FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_ldcf_features rows");
     const size_t cells = fr_cells(s->dim, N);
     uint32_t *hist = nullptr;
-    fr_status st = fr_alloc(ctx, (void **)&hist, cells * sizeof(uint32_t));
+    fr_status st = fr_alloc(ctx, (void **)&hist, (cells + 1) * sizeof(uint32_t));
     if (st) return st;
     hipStream_t q = ctx->stream;
-    hipError_t e = hipMemsetAsync(hist, 0, cells * sizeof(uint32_t), q);
+    hipError_t e = hipMemsetAsync(hist, 0, (cells + 1) * sizeof(uint32_t), q);
     if (e == hipSuccess && s->M > 0) {
         launch_counts(q, s, N, hist);
         hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist,
@@ -713,12 +819,12 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
         hipFree(m->hist);
         m->hist = nullptr;
         m->hist_cells = 0;
-        fr_status st = fr_alloc(ctx, (void **)&m->hist, cells * sizeof(uint32_t));
+        fr_status st = fr_alloc(ctx, (void **)&m->hist, (cells + 1) * sizeof(uint32_t));
         if (st) return st;
         m->hist_cells = cells;
     }
     if (s->M > 0) {
-        FR_HIP(ctx, hipMemsetAsync(m->hist, 0, cells * sizeof(uint32_t), ctx->stream));
+        FR_HIP(ctx, hipMemsetAsync(m->hist, 0, (cells + 1) * sizeof(uint32_t), ctx->stream));
         launch_counts(ctx->stream, s, N, m->hist);
         const int64_t groups = (s->M + 63) / 64;
         const int64_t waves = std::min<int64_t>(groups, (int64_t)std::max(ctx->cu_count, 1) * 32);

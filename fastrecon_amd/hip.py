@@ -36,6 +36,7 @@ EXPORTS = [
     "fr_fftshift", "fr_sos", "fr_fft_sos", "fr_deapod_map", "fr_lowfilter", "fr_normalize", "fr_crop", "fr_sos_map",
     "fr_grids_copy",
     "fr_dcf_pipe_menon",
+    "fr_dcf_identity_scale",
     "fr_graph_create", "fr_graph_launch", "fr_graph_destroy", "fr_autotune",
     "fr_ldcf_features", "fr_mlp_param_count", "fr_mlp_create", "fr_mlp_destroy", "fr_mlp_set_params",
     "fr_mlp_get_params", "fr_mlp_forward", "fr_mlp_dcf", "fr_ldcf_dcf", "fr_mlp_grad", "fr_mlp_train_step",
@@ -116,6 +117,7 @@ def load():
         "fr_fft_sos": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp],
         "fr_deapod_map": [_vp, C.c_int, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
         "fr_dcf_pipe_menon": [_vp, _vp, C.c_int, C.c_int],
+        "fr_dcf_identity_scale": [_vp, _vp, C.POINTER(C.c_float)],
         "fr_lowfilter": [_vp, _vp, C.c_int],
         "fr_normalize": [_vp, _vp],
         "fr_crop": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
@@ -483,6 +485,14 @@ def deapod_map(ctx, dim, N, kwidth, lut, centre, rc):
 def pipe_menon(plan, samples, iters=10, init_ones=True):
     plan.ctx.check(plan.ctx.L.fr_dcf_pipe_menon(plan.h, samples.h, iters, 1 if init_ones else 0),
                    "fr_dcf_pipe_menon")
+
+
+def dcf_identity_scale(plan, samples):
+    """Scale the samples' DCF in place by alpha = sum(w) / |G w|^2 (the
+    Pipe-Menon fixed-point identity); returns alpha."""
+    a = C.c_float(0.0)
+    plan.ctx.check(plan.ctx.L.fr_dcf_identity_scale(plan.h, samples.h, C.byref(a)), "fr_dcf_identity_scale")
+    return float(a.value)
 
 
 def lowfilter(ctx, grids, res):

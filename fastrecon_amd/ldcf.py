@@ -1,9 +1,10 @@
 """Learned density compensation (net-new: the reference has no DCF
 estimation).
 
-A 6-64-64-1 SiLU MLP maps per-sample trajectory features (radius, azimuth,
-polar angle, neighbour counts within 1/2/4 cells; computed on the device by
-fr_ldcf_features) to log DCF.  Its targets are 10-iteration Pipe-Menon DCFs
+An 8-64-64-1 SiLU MLP maps per-sample trajectory features (radius, azimuth,
+polar angle, neighbour counts within 1/2/4 cells, distance to the outermost
+sample shell, same-cell count; computed on the device by fr_ldcf_features)
+to log DCF.  Its targets are 10-iteration Pipe-Menon DCFs
 that this package's own gridding and degridding kernels produce
 (fr_dcf_pipe_menon), on a seeded set of variable-density 3D radial and cones
 trajectories for a 128^3 image (256^3 grid).  Training runs on the device
@@ -153,7 +154,7 @@ def build_dataset(ctx, count=200, per_traj=50000, scale=1.0, seed=7, iters=10, l
 def _init_params(seed):
     """He-style initialisation of the flat parameter vector (FP32)."""
     rng = np.random.default_rng(seed)
-    nf, h = 6, 64
+    nf, h = 8, 64
     W1 = rng.standard_normal((nf, h)) * np.sqrt(2.0 / nf)
     W2 = rng.standard_normal((h, h)) * np.sqrt(2.0 / h)
     w3 = rng.standard_normal(h) * np.sqrt(1.0 / h)
@@ -208,10 +209,13 @@ def load(ctx, path=CHECKPOINT):
     return mlp, json.loads(str(z["meta"]))
 
 
-def learned_dcf(ctx, mlp, samples, N):
+def learned_dcf(ctx, mlp, samples, N, scale_plan=None):
     """Features of the samples' (normalised) trajectory -> MLP -> samples' DCF
-    (one fused pass, fr_ldcf_dcf)."""
+    (one fused pass, fr_ldcf_dcf); with `scale_plan`, then the global scale
+    of the Pipe-Menon identity (one single-channel gridding pass on that
+    plan, fr_dcf_identity_scale).  Returns the scale applied (1.0 without)."""
     mlp.dcf_from_traj(samples, N)
+    return hip.dcf_identity_scale(scale_plan, samples) if scale_plan is not None else 1.0
 
 
 def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None, coils=None):
@@ -219,7 +223,10 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     upload): end-to-end milliseconds of each recon from the plan (iterative:
     Pipe-Menon + grid + FFT + SOS, on the recon's plan and on a second plan
     shaped for single-channel gridding; learned: features + MLP + grid + FFT +
-    SOS), image NRMSE of learned (and ramp) against iterative, PSF sidelobes.
+    SOS; learned + identity scale: one single-channel gridding pass more),
+    the plan build (ratios with it included: a fresh trajectory pays it on
+    both sides), image NRMSE of learned (and ramp) against iterative, PSF
+    sidelobes.
     Times with torch's HIP events, so torch must have been imported before
     the library was loaded (the binding's module docstring says why)."""
     import torch  # device timing through HIP events on the context stream
@@ -227,7 +234,10 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     tn, _ = hip.normalize_traj(traj, N)
     M = tn.shape[0]
     C = coils if callable(data) else data.shape[0]
+    t0 = time.perf_counter()
     s, p = plan_for(ctx, tn, N, C)
+    ctx.sync()
+    t_plan = (time.perf_counter() - t0) * 1e3
     if callable(data):
         data(s)                  # the caller fills the coil data (e.g. on the device)
     else:
@@ -278,6 +288,17 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     t_mr = timed(learned_refined)
     im_mr = img.to_host(np.float32)
 
+    def learned_scaled():
+        # the learned DCF with its global scale from the Pipe-Menon identity
+        mlp.dcf_from_traj(s, N)
+        hip.dcf_identity_scale(p, s)
+        recon()
+
+    t_ms = timed(learned_scaled)
+    im_ms = img.to_host(np.float32)
+    mlp.dcf_from_traj(s, N)
+    alpha = hip.dcf_identity_scale(p, s)     # the scale the identity gives the learned DCF
+
     # the iterative DCF on a second plan of the same samples with the unit
     # shape that single-channel gridding prefers (the default shape for one
     # coil), then the same recon: the fastest iterative path this library has
@@ -296,7 +317,12 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     hip.pipe_menon(p, s, 1, True)
     recon()
     im_pm1 = img.to_host(np.float32)
-    out = dict(ms_iterative=t_it, ms_learned=t_ml, ms_recon_only=t_rc, speedup=t_it / t_ml,
+    out = dict(ms_plan=t_plan, ms_iterative=t_it, ms_learned=t_ml, ms_recon_only=t_rc, speedup=t_it / t_ml,
+               speedup_with_plan=(t_plan + t_it) / (t_plan + t_ml),
+               ms_learned_scaled=t_ms, speedup_learned_scaled=t_it / t_ms,
+               speedup_learned_scaled_with_plan=(t_plan + t_it) / (t_plan + t_ms),
+               learned_identity_scale=alpha,
+               nrmse_learned_identity_scaled=nrmse(im_ms, im_it),
                nrmse_pm1=nrmse(im_pm1, im_it), nrmse_pm1_scaled=nrmse_scaled(im_pm1, im_it),
                nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
                ms_learned_pm1=t_mr, speedup_learned_pm1=t_it / t_mr,
@@ -315,16 +341,6 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
     ones.set_coils(np.ones((1, M), np.complex64))
     g1 = hip.Grids(ctx, tn.shape[1], N, 1)
     f1 = hip.FFT(ctx, tn.shape[1], N, 1, +1)
-    # global scale from the Pipe-Menon fixed-point identity |G w|^2 = sum(w)
-    # (G^H G w = 1 at convergence): alpha = sum(w) / |G w|^2 for the learned w
-    ones.set_dcf(w_ml)
-    hip.grid_execute(p, ones, g1)
-    gw = g1.download(0)
-    alpha = float(np.sum(w_ml, dtype=np.float64) / np.sum(np.abs(gw.astype(np.complex128)) ** 2))
-    s.set_dcf((alpha * w_ml).astype(np.float32))
-    recon()
-    im_al = img.to_host(np.float32)
-    out.update(learned_identity_scale=alpha, nrmse_learned_identity_scaled=nrmse(im_al, im_it))
     for name, w in (("iterative", w_it), ("learned", w_ml)) + ((("ramp", ramp),) if ramp is not None else ()):
         ones.set_dcf(w)
         psf = recon_image(ctx, p, ones, N, rc, f1, g1)
@@ -384,7 +400,10 @@ def evaluate_phantom(ctx, mlp, traj, N, rc, image, ramp=None, iters=10):
     im_it = recon_image(ctx, p, s, N, rc, f, g)
     learned_dcf(ctx, mlp, s, N)
     im_ml = recon_image(ctx, p, s, N, rc, f, g)
-    out = dict(nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it))
+    hip.dcf_identity_scale(p, s)
+    im_mi = recon_image(ctx, p, s, N, rc, f, g)
+    out = dict(nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
+               nrmse_learned_identity_scaled=nrmse(im_mi, im_it))
     if ramp is not None:
         s.set_dcf(ramp)
         im_rp = recon_image(ctx, p, s, N, rc, f, g)

@@ -48,7 +48,7 @@ def main():
     meta = dict(count=a.count, per_traj=a.per_traj, scale=a.scale, steps=a.steps, batch=a.batch, lr=a.lr,
                 seed=a.seed, rows=int(X.shape[0]), dataset_s=round(t_data, 1), train_s=round(t_train, 1),
                 losses=[(int(s), float(l)) for s, l in losses], grid=ldcf.N_GRID, image=ldcf.IMAGE,
-                targets="10-iteration Pipe-Menon DCF", features="|u|, azimuth, polar, log1p(n_1,2,4)/8",
+                targets="10-iteration Pipe-Menon DCF", features="|u|, azimuth, polar, log1p(n_1,2,4)/8, outer-shell distance, log1p(n_0)/8",
                 row_strata="1/2 radius-stratified, 1/4 log-radius-stratified, 1/4 drawn ~ w^2")
     ldcf.save(net, a.out, meta)
     log(f"[ldcf] trained in {t_train:.1f} s, final loss {losses[-1][1]:.5f}; wrote {a.out}")

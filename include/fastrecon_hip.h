@@ -271,6 +271,13 @@ fr_status fr_deapod_map(fr_ctx *ctx, int dim, unsigned N, float kWidth, const fl
  * to the samples' DCF.  Uses this plan's gridding and degridding kernels. */
 fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones);
 
+/* Global scale of the samples' DCF (net-new) from the Pipe-Menon fixed-point
+ * identity sum(w) = |G w|^2 (G^H G w = 1 at convergence): one single-channel
+ * gridding of w on this plan, then w <- alpha w with alpha = sum(w)/|G w|^2,
+ * exact when w is the fixed point times a constant.  alpha_out (host, may be
+ * NULL) receives alpha.  Not for batched frames. */
+fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, float *alpha_out);
+
 /* ------------------------------------------------- graphs and autotuning
  * Net-new (the reference launches per coil and partition with a device
  * synchronisation after each, cuGridLut.cu:261-309).  fr_graph_create runs
@@ -297,10 +304,12 @@ fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth
 
 /* --------------------------------------------- learned density compensation
  * Net-new (the reference has no DCF estimation).  Features per sample, 8
- * floats per row (6 used): log1p(|u| N/2)/5, azimuth/pi, polar/pi, log1p(n_1)/8,
- * log1p(n_2)/8, log1p(n_4)/8, with u = (t - (N-1)/2)/(N/2) for the samples'
- * normalised trajectory t and n_r the number of other samples whose nearest
- * cell lies within Chebyshev distance r cells.  feat_dev: M*8 floats. */
+ * floats per row: log1p(rho)/5, azimuth/pi, polar/pi, log1p(n_1)/8,
+ * log1p(n_2)/8, log1p(n_4)/8, min(rho_max - rho, 4)/4, log1p(n_0)/8, with
+ * u = (t - (N-1)/2)/(N/2) for the samples' normalised trajectory t,
+ * rho = |u| N/2 (rho_max its largest value over the samples) and n_r the
+ * number of other samples whose nearest cell lies within Chebyshev distance
+ * r cells.  feat_dev: M*8 floats. */
 fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev);
 
 /* MLP 6 -> 64 -> 64 -> 1 (SiLU), BF16 weight matrices, FP32 biases and

@@ -13,6 +13,9 @@ ReconData::normalizeTraj, ReconData.cpp:211-225), with cf = (N-1)/2, h = N/2:
   cell  = clamp(floor(t_d + 0.5), 0, N-1)
   n_r   = #{samples j != i : max_d |cell_j - cell_i| <= r},  r = 1, 2, 4
   f3..5 = log1p(n_r) / 8
+  rho   = |u| N/2 (FP32: sqrt, then times N/2), rho_max = max over samples
+  f6    = min(rho_max - rho, 4) / 4           distance to the outermost shell
+  f7    = log1p(n_0) / 8                      other samples in the same cell
 The counts are pinned by brute force O(M^2) here.
 
 MLP 6 -> 64 -> 64 -> 1 with SiLU, BF16 weight matrices, FP32 biases:
@@ -24,7 +27,7 @@ differences of the same function in FP64 without rounding.
 """
 import numpy as np
 
-NF = 6
+NF = 8
 H = 64
 
 
@@ -61,6 +64,10 @@ def features(tn, N):
         for i in range(M):
             n[i] = np.count_nonzero(np.max(np.abs(cell - cell[i]), axis=1) <= rr) - 1
         f[:, 3 + q] = np.log1p(n) / 8.0
+    rho = (r * h).astype(np.float32)
+    f[:, 6] = np.minimum((rho.max() - rho).astype(np.float32), np.float32(4.0)).astype(np.float64) * 0.25
+    n0 = np.array([np.count_nonzero(np.all(cell == cell[i], axis=1)) - 1 for i in range(M)])
+    f[:, 7] = np.log1p(n0) / 8.0
     return f
 
 
@@ -80,7 +87,7 @@ def dsilu(z):
 
 
 def unpack(p):
-    """Flat parameter vector (the C ABI's order) -> W1 (6,64), b1, W2 (64,64), b2, w3 (64), b3."""
+    """Flat parameter vector (the C ABI's order) -> W1 (NF,64), b1, W2 (64,64), b2, w3 (64), b3."""
     p = np.asarray(p, np.float64)
     o = 0
     W1 = p[o:o + NF * H].reshape(NF, H); o += NF * H

@@ -324,6 +324,12 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     for q, rr in enumerate((1, 2, 4)):
         n = np.asarray(tree.query_ball_point(cells[idx], rr + 0.5, p=np.inf, return_length=True)) - 1
         f[:, 3 + q] = np.log1p(n) / 8.0
+    ua = (tn - cf_) / h
+    rho_all = (np.sqrt((ua[:, 0] * ua[:, 0] + ua[:, 1] * ua[:, 1]) + ua[:, 2] * ua[:, 2]) * h).astype(np.float32)
+    rho = (r * h).astype(np.float32)
+    f[:, 6] = np.minimum(rho_all.max() - rho, np.float32(4.0)).astype(np.float64) * 0.25
+    n0 = np.asarray(tree.query_ball_point(cells[idx], 0.5, p=np.inf, return_length=True)) - 1
+    f[:, 7] = np.log1p(n0) / 8.0
     params, mu, sigma = mlp.get_params()
     y, _ = lo.forward(params, f.astype(np.float32), emulate=True)
     w_ref = np.exp(mu + sigma * y.astype(np.float64))

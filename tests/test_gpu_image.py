@@ -144,6 +144,29 @@ def test_pipe_menon_matches_oracle(ctx, orc, dim):
     assert e <= 1e-6          # ten FP32 iterations of a ratio update (measured ~1.5e-7)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dcf_identity_scale(ctx, orc, dim):
+    """fr_dcf_identity_scale: alpha = sum(w) / |G w|^2 against the FP64
+    oracle's gridding of w, the in-place scaling bitwise w * alpha in FP32,
+    and a second call on the scaled DCF returns 1 (the identity holds)."""
+    if dim == 2:
+        tn, dcf, data, N, lut, s, p, g = setup(ctx, 2, 60, 128, 1, 64, 3)
+    else:
+        tn, dcf, data, N, lut, s, p, g = setup(ctx, 3, 300, 32, 1, 16, 3)
+    hip.pipe_menon(p, s, iters=3, init_ones=True)
+    w = (s.get_dcf() * np.float32(2.5)).astype(np.float32)
+    s.set_dcf(w)
+    a = hip.dcf_identity_scale(p, s)
+    g64 = orc.grid_f64(tn, w, np.ones((1, tn.shape[0]), np.complex64), N, 4.0, lut, nthreads=8)[0]
+    ref = float(np.sum(w, dtype=np.float64) / np.sum(np.abs(g64) ** 2))
+    got = s.get_dcf()
+    a2 = hip.dcf_identity_scale(p, s)
+    print(f"[identity scale dim={dim}] alpha {a:.6f} (FP64 oracle {ref:.6f}), second call {a2:.7f}", flush=True)
+    assert abs(a / ref - 1.0) <= 1e-5
+    assert np.array_equal(got.view(np.int32), (w * np.float32(a)).astype(np.float32).view(np.int32))
+    assert abs(a2 - 1.0) <= 1e-5
+
+
 @pytest.mark.parametrize("dim,N,C", [(2, 64, 4), (3, 32, 3)])
 def test_pils_ops_bitwise(ctx, orc, dim, N, C):
     """lowFilter, normalize, crop and SOS(map) are element-wise FP32 with the

@@ -43,13 +43,17 @@ def test_features_match_oracle(ctx, dim, N, M):
     s = samples_for(ctx, t)
     f = hip.ldcf_features(ctx, s, N).to_host(np.float32).reshape(M, 8)
     ref = lo.features(t, N)
-    print(f"[features dim={dim}] max |df| = {np.abs(f[:, :6] - ref).max(axis=0)}", flush=True)
+    print(f"[features dim={dim}] max |df| = {np.abs(f - ref).max(axis=0)}", flush=True)
     for q, r in enumerate((1, 2, 4)):
         n_gpu = np.rint(np.expm1(f[:, 3 + q].astype(np.float64) * 8)).astype(np.int64)
         assert np.array_equal(n_gpu, lo.neighbour_counts(t, N, r))
     np.testing.assert_allclose(f[:, 0], ref[:, 0], rtol=2e-6, atol=1e-6)
     np.testing.assert_allclose(f[:, 1:3], ref[:, 1:3], atol=1e-5)
-    assert np.all(f[:, 6:] == 0)
+    # outer-shell distance: exact FP32 radii and maximum
+    np.testing.assert_allclose(f[:, 6], ref[:, 6], rtol=0, atol=2e-6)
+    assert np.sum(f[:, 6] == 0.0) >= 1
+    n0 = np.rint(np.expm1(f[:, 7].astype(np.float64) * 8)).astype(np.int64)
+    assert np.array_equal(n0, np.rint(np.expm1(ref[:, 7] * 8)).astype(np.int64))
 
 
 @pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41), (3, 6, 500)])
@@ -94,12 +98,12 @@ def test_forward_matches_oracle(ctx):
     rng = np.random.default_rng(3)
     M = 5000 + 7                     # not a multiple of 16
     X = np.zeros((M, 8), np.float32)
-    X[:, :6] = rng.uniform(-1.5, 1.5, (M, 6))
+    X[:, :] = rng.uniform(-1.5, 1.5, (M, 8))
     p = random_net(3)
     net = hip.MLP(ctx)
     net.set_params(p, 0.0, 1.0)
     y = net.forward(upload(ctx, X), M).to_host(np.float32)
-    ref, _ = lo.forward(p, X[:, :6], emulate=True)
+    ref, _ = lo.forward(p, X, emulate=True)
     d = np.abs(y - ref)
     print(f"[mlp forward] max |dy| {d.max():.2e}, median {np.median(d):.2e}", flush=True)
     assert np.median(d) <= 1e-5 and d.max() <= 1e-2
@@ -115,7 +119,7 @@ def test_gradient_matches_oracle_and_is_deterministic(ctx):
     rng = np.random.default_rng(4)
     B = 4096 + 5
     X = np.zeros((B, 8), np.float32)
-    X[:, :6] = rng.uniform(-1.5, 1.5, (B, 6))
+    X[:, :] = rng.uniform(-1.5, 1.5, (B, 8))
     t = rng.standard_normal(B).astype(np.float32)
     p = random_net(4)
     net = hip.MLP(ctx)
@@ -124,7 +128,7 @@ def test_gradient_matches_oracle_and_is_deterministic(ctx):
     g1, l1 = net.grad(fx, ft, B)
     g2, l2 = net.grad(fx, ft, B)
     assert np.array_equal(g1, g2) and l1 == l2
-    Lr, gr = lo.loss_grad(p, X[:, :6], t, emulate=True)
+    Lr, gr = lo.loss_grad(p, X, t, emulate=True)
     e = np.linalg.norm(g1 - gr) / np.linalg.norm(gr)
     print(f"[mlp grad] loss {l1:.6f} vs {Lr:.6f}, grad rel-L2 {e:.2e}", flush=True)
     assert abs(l1 - Lr) <= 1e-4 * abs(Lr) and e <= 1e-3
@@ -140,7 +144,7 @@ def test_training_reduces_loss(ctx):
     rng = np.random.default_rng(5)
     n = 65536
     X = np.zeros((n, 8), np.float32)
-    X[:, :6] = rng.uniform(0, 1, (n, 6))
+    X[:, :] = rng.uniform(0, 1, (n, 8))
     W = np.exp(2.0 * X[:, 0] ** 2 - X[:, 3] + 0.5 * np.sin(3 * X[:, 1]))
     net, mu, sigma, losses = ldcf.train(ctx, X, W, steps=400, batch=8192, lr=3e-3, log=None, log_every=100)
     print(f"[ldcf train] losses {losses}", flush=True)

@@ -76,4 +76,4 @@ def test_checkpoint_metadata():
     meta = json.loads(str(z["meta"]))
     assert isinstance(meta["rows"], int) and meta["rows"] > 0
     assert meta["steps"] > 0 and meta["losses"][-1][1] > 0
-    assert z["params"].dtype == np.float32 and z["params"].size == 6 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    assert z["params"].dtype == np.float32 and z["params"].size == 8 * 64 + 64 + 64 * 64 + 64 + 64 + 1

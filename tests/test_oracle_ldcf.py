@@ -21,7 +21,7 @@ def test_bf16_round_known_values():
 def test_gradient_matches_finite_differences(seed):
     rng = np.random.default_rng(seed)
     p = lo.init_params(seed).astype(np.float64) + 0.05 * rng.standard_normal(lo.NPARAM)
-    x = rng.standard_normal((40, 6))
+    x = rng.standard_normal((40, lo.NF))
     t = rng.standard_normal(40)
     L, g = lo.loss_grad(p, x, t, emulate=False)
     # check a spread of parameters of every tensor
@@ -45,6 +45,9 @@ def test_neighbour_counts_match_kdtree(dim, N, M):
     for r in (1, 2, 4):
         ref = np.array([len(q) - 1 for q in tree.query_ball_point(cell, r + 0.5, p=np.inf)])
         assert np.array_equal(lo.neighbour_counts(t, N, r), ref)
+    # f7: the other samples in the same cell (Chebyshev radius 0)
+    same = np.array([len(q) - 1 for q in tree.query_ball_point(cell, 0.5, p=np.inf)])
+    np.testing.assert_allclose(lo.features(t, N)[:, 7], np.log1p(same) / 8, rtol=1e-12)
 
 
 def test_angular_features_closed_forms():
@@ -56,7 +59,12 @@ def test_angular_features_closed_forms():
     np.testing.assert_allclose(f[:, 0], np.log1p([8, 8, 8, 8, np.sqrt(2) * 8]) / 5, rtol=1e-6)
     np.testing.assert_allclose(f[:, 1], [0.0, 0.5, 0.0, -0.5, 1.0], atol=1e-7)
     np.testing.assert_allclose(f[:, 2], [0.5, 0.5, 0.0, 0.5, 0.75], atol=1e-7)
-    assert np.all(f[:, 3:] == 0)           # isolated points: no neighbours
+    assert np.all(f[:, 3:6] == 0) and np.all(f[:, 7] == 0)    # isolated points: no neighbours
+    # distance to the outermost shell: 8 sqrt(2) - 8 cells for the four at radius 8
+    np.testing.assert_allclose(f[:, 6], [(np.sqrt(2) - 1) * 2] * 4 + [0.0], atol=2e-6)
+    # clipped at 4 cells
+    g = lo.features(np.array([[c + 2, c, c], [c, c + 12, c]], np.float32), N)
+    np.testing.assert_allclose(g[:, 6], [1.0, 0.0], atol=1e-7)
 
 
 def test_training_rows_strata():
