@@ -5,6 +5,7 @@
 #include "fr_internal.h"
 #include "fr_taps.h"
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -187,63 +188,86 @@ extern "C" fr_status fr_degrid_execute(fr_plan *p, const fr_samples *s, const fr
     return fr_degrid_raw(p, s, g->data, g->C, (float2 *)out_dev);
 }
 
+// The plan's single-channel workspace, created on first use: unit data
+// sharing the samples' trajectory (copied on every call, since the caller's
+// samples may be another handle on the same trajectory), a one-channel grid
+// and the degridded densities.  Its DCF is the caller's to fill.
+static fr_status dcf_workspace(fr_plan *p, const fr_samples *s)
+{
+    fr_ctx *ctx = p->ctx;
+    const int64_t M = s->M;
+    fr_status st = FR_OK;
+    if (!p->dcf_u) {
+        if (s->frames > 0) {
+            std::vector<int64_t> cnt(s->frames);
+            for (int f = 0; f < s->frames; ++f) cnt[f] = s->frame_off[f + 1] - s->frame_off[f];
+            st = fr_samples_create_frames(ctx, s->frames, cnt.data(), 1, FR_SAMPLES_C64, &p->dcf_u);
+            if (!st) st = fr_grids_create_frames(ctx, p->N, 1, s->frames, &p->dcf_g);
+        } else {
+            st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &p->dcf_u);
+            if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &p->dcf_g);
+        }
+        // (the identity scale keeps its partial sums in the same buffer)
+        const size_t den_bytes = std::max(sizeof(float2) * (size_t)M, sizeof(double) * (2 * SUM_BLOCKS + 1));
+        if (!st) st = fr_alloc(ctx, (void **)&p->dcf_den, den_bytes);
+        if (st) {
+            hipFree(p->dcf_den);
+            fr_grids_destroy(p->dcf_g);
+            fr_samples_destroy(p->dcf_u);
+            p->dcf_den = nullptr;
+            p->dcf_g = nullptr;
+            p->dcf_u = nullptr;
+            return st;
+        }
+        hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, (float2 *)p->dcf_u->data, M);
+        FR_HIP(ctx, hipGetLastError());
+    }
+    const int rows = s->frames > 0 ? 3 : s->dim;
+    FR_HIP(ctx, hipMemcpyAsync(p->dcf_u->traj, s->traj, sizeof(float) * rows * M, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    p->dcf_u->has_traj = p->dcf_u->has_dcf = true;
+    return FR_OK;
+}
+
+// the single-channel gridding mode of a plan: the matrix-core kernels
+// (the records and taps of plans in modes 1, 2 and 5 serve it; several
+// times faster here than the LDS kernels, whose lanes are the coil
+// components), a position-weight plan's own mode, an exact-order plan's
+// sequential mode (CPU summation order)
+static int density_mode(const fr_plan *p)
+{
+    return p->accumulate == FR_ACC_POS ? FR_ACC_POS
+                                       : (p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
+}
+
 extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones)
 {
     if (!p || !s) return FR_ERR_INVALID;
     fr_ctx *ctx = p->ctx;
     FR_CHECK(ctx, s->M == p->M && s->dim == p->dim, FR_ERR_INVALID, "samples do not match the plan");
     FR_CHECK(ctx, iters >= 0, FR_ERR_INVALID, "iters must be >= 0");
-    const int64_t M = s->M;
-    // single-channel unit data sharing the trajectory; its DCF is the weight
-    fr_samples *u = nullptr;
-    fr_grids *g = nullptr;
-    float2 *den = nullptr;
     FR_CHECK(ctx, s->frames == p->frames, FR_ERR_INVALID, "samples and plan differ in frames");
-    fr_status st;
-    int rows = s->dim;
-    if (s->frames > 0) {
-        std::vector<int64_t> cnt(s->frames);
-        for (int f = 0; f < s->frames; ++f) cnt[f] = s->frame_off[f + 1] - s->frame_off[f];
-        st = fr_samples_create_frames(ctx, s->frames, cnt.data(), 1, FR_SAMPLES_C64, &u);
-        if (!st) st = fr_grids_create_frames(ctx, p->N, 1, s->frames, &g);
-        rows = 3;
-    } else {
-        st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &u);
-        if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &g);
+    const int64_t M = s->M;
+    fr_status st = dcf_workspace(p, s);
+    if (st) return st;
+    fr_samples *u = p->dcf_u;
+    if (init_ones || !s->has_dcf)
+        hipLaunchKernelGGL(k_fill_f32, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, M, 1.f);
+    else
+        hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+    const int mode = density_mode(p);
+    for (int it = 0; it < iters && !st; ++it) {
+        st = fr_launch_grid(p, u, p->dcf_g->data, mode);
+        if (!st) st = fr_degrid_raw(p, u, p->dcf_g->data, 1, p->dcf_den);
+        if (!st)
+            hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, p->dcf_den, M);
     }
-    if (!st) st = fr_alloc(ctx, (void **)&den, sizeof(float2) * M);
-    if (!st) {
-        hipMemcpyAsync(u->traj, s->traj, sizeof(float) * rows * M, hipMemcpyDeviceToDevice, ctx->stream);
-        hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, (float2 *)u->data, M);
-        if (init_ones || !s->has_dcf)
-            hipLaunchKernelGGL(k_fill_f32, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, M, 1.f);
-        else
-            hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
-        u->has_traj = u->has_dcf = true;
-        for (int it = 0; it < iters && !st; ++it) {
-            // single-channel density gridding: the matrix-core kernel (the
-            // records and taps of plans in modes 1, 2 and 5 serve it), several
-            // times faster here than the LDS kernels, whose lanes are the coil
-            // components; an exact-order plan keeps its sequential mode (CPU
-            // summation order)
-            // (a position-weight plan stores no taps: it keeps its own mode)
-            const int mode = p->accumulate == FR_ACC_POS ? FR_ACC_POS
-                             : (p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
-            st = fr_launch_grid(p, u, g->data, mode);
-            if (!st) st = fr_degrid_raw(p, u, g->data, 1, den);
-            if (!st) hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, den, M);
-        }
-        if (!st) {
-            hipMemcpyAsync(s->dcf, u->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
-            s->has_dcf = true;
-            hipError_t e = hipStreamSynchronize(ctx->stream);
-            if (e != hipSuccess) st = fr_fail(ctx, FR_ERR_HIP, "pipe-menon: %s", hipGetErrorString(e));
-        }
-    }
-    hipFree(den);
-    fr_grids_destroy(g);
-    fr_samples_destroy(u);
-    return st;
+    if (st) return st;
+    hipMemcpyAsync(s->dcf, u->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+    s->has_dcf = true;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    FR_CHECK(ctx, e == hipSuccess, FR_ERR_HIP, "pipe-menon: %s", hipGetErrorString(e));
+    return FR_OK;
 }
 
 // Global scale of a DCF from the Pipe-Menon fixed-point identity: at
@@ -261,36 +285,24 @@ extern "C" fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, float *alp
              "identity scale of batched frames: scale each frame on its own plan");
     FR_CHECK(ctx, s->has_traj && s->has_dcf, FR_ERR_STATE, "samples need a trajectory and a DCF");
     const int64_t M = s->M;
-    fr_samples *u = nullptr;
-    fr_grids *g = nullptr;
-    double *part = nullptr;
-    float *alpha = nullptr;
-    fr_status st = fr_samples_create(ctx, s->dim, M, 1, FR_SAMPLES_C64, &u);
-    if (!st) st = fr_grids_create(ctx, s->dim, p->N, 1, &g);
-    if (!st) st = fr_alloc(ctx, (void **)&part, sizeof(double) * 2 * SUM_BLOCKS + sizeof(float));
-    if (!st) {
-        alpha = (float *)(part + 2 * SUM_BLOCKS);
-        hipMemcpyAsync(u->traj, s->traj, sizeof(float) * s->dim * M, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
-        hipLaunchKernelGGL(k_fill_unit, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, (float2 *)u->data, M);
-        u->has_traj = u->has_dcf = true;
-        const int mode = p->accumulate == FR_ACC_POS ? FR_ACC_POS
-                         : (p->accumulate != FR_ACC_GATHER && !p->exact_order ? FR_ACC_MFMA : 0);
-        st = fr_launch_grid(p, u, g->data, mode);
-    }
-    if (!st) {
-        hipLaunchKernelGGL(k_identity_partials, dim3(SUM_BLOCKS), dim3(256), 0, ctx->stream, g->data,
-                           (int64_t)g->cells, s->dcf, M, part);
-        hipLaunchKernelGGL(k_identity_alpha, dim3(1), dim3(64), 0, ctx->stream, part, SUM_BLOCKS, alpha);
-        hipLaunchKernelGGL(k_scale_by, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, s->dcf, M, alpha);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess && alpha_out)
-            e = hipMemcpyAsync(alpha_out, alpha, sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) st = fr_fail(ctx, FR_ERR_HIP, "identity scale: %s", hipGetErrorString(e));
-    }
-    hipFree(part);
-    fr_grids_destroy(g);
-    fr_samples_destroy(u);
-    return st;
+    fr_status st = dcf_workspace(p, s);
+    if (st) return st;
+    fr_samples *u = p->dcf_u;
+    hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+    st = fr_launch_grid(p, u, p->dcf_g->data, density_mode(p));
+    if (st) return st;
+    // the partial sums and alpha live in the degridded-density buffer (sized
+    // for them)
+    double *part = reinterpret_cast<double *>(p->dcf_den);
+    float *alpha = reinterpret_cast<float *>(part + 2 * SUM_BLOCKS);
+    hipLaunchKernelGGL(k_identity_partials, dim3(SUM_BLOCKS), dim3(256), 0, ctx->stream, p->dcf_g->data,
+                       (int64_t)p->dcf_g->cells, s->dcf, M, part);
+    hipLaunchKernelGGL(k_identity_alpha, dim3(1), dim3(64), 0, ctx->stream, part, SUM_BLOCKS, alpha);
+    hipLaunchKernelGGL(k_scale_by, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, s->dcf, M, alpha);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && alpha_out)
+        e = hipMemcpyAsync(alpha_out, alpha, sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    FR_CHECK(ctx, e == hipSuccess, FR_ERR_HIP, "identity scale: %s", hipGetErrorString(e));
+    return FR_OK;
 }

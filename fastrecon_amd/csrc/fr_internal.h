@@ -134,6 +134,12 @@ struct fr_heavy {
 
 struct fr_plan {
     fr_ctx *ctx = nullptr;
+    // single-channel workspace of Pipe-Menon and the identity scale (unit
+    // data on the plan's trajectory, a one-channel grid, degridded
+    // densities): created on first use, kept until the plan is destroyed
+    fr_samples *dcf_u = nullptr;
+    fr_grids *dcf_g = nullptr;
+    float2 *dcf_den = nullptr;
     int dim = 0;
     unsigned N = 0;
     int64_t M = 0;

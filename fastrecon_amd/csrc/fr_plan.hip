@@ -938,6 +938,9 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
     hipFree(p->lut); hipFree(p->units); hipFree(p->hdrs); hipFree(p->taps); hipFree(p->tstart);
     hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches); hipFree(p->mfma_pairs);
     hipFree(p->recs); hipFree(p->pos); hipFree(p->tsel); hipFree(p->tiles);
+    hipFree(p->dcf_den);
+    fr_grids_destroy(p->dcf_g);
+    fr_samples_destroy(p->dcf_u);
     delete p;
     return FR_OK;
 }

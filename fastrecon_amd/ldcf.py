@@ -335,6 +335,10 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
         recon()
         im_rp = img.to_host(np.float32)
         out.update(nrmse_ramp=nrmse(im_rp, im_it), nrmse_ramp_scaled=nrmse_scaled(im_rp, im_it))
+        # the ramp with the same identity scale (no reference image needed)
+        hip.dcf_identity_scale(p, s)
+        recon()
+        out.update(nrmse_ramp_identity_scaled=nrmse(img.to_host(np.float32), im_it))
     # PSF: all-ones data with each DCF, one channel
     ones = hip.Samples(ctx, tn.shape[1], M, 1)
     ones.set_traj(tn)
@@ -408,6 +412,8 @@ def evaluate_phantom(ctx, mlp, traj, N, rc, image, ramp=None, iters=10):
         s.set_dcf(ramp)
         im_rp = recon_image(ctx, p, s, N, rc, f, g)
         out.update(nrmse_ramp=nrmse(im_rp, im_it), nrmse_ramp_scaled=nrmse_scaled(im_rp, im_it))
+        hip.dcf_identity_scale(p, s)
+        out.update(nrmse_ramp_identity_scaled=nrmse(recon_image(ctx, p, s, N, rc, f, g), im_it))
     for h in (g, f, p, s):
         h.close()
     return out

@@ -268,7 +268,10 @@ fr_status fr_deapod_map(fr_ctx *ctx, int dim, unsigned N, float kWidth, const fl
 /* --------------------------------------------------- density compensation */
 /* Pipe-Menon iterative DCF (net-new): w <- w / G^H G w, `iters` times,
  * starting from the samples' current DCF (or 1 if init_ones), written back
- * to the samples' DCF.  Uses this plan's gridding and degridding kernels. */
+ * to the samples' DCF.  Uses this plan's gridding and degridding kernels.
+ * The first call (of this or fr_dcf_identity_scale) gives the plan a
+ * single-channel workspace (the trajectory, unit data, one grid, M complex
+ * densities) that it keeps until fr_plan_destroy. */
 fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones);
 
 /* Global scale of the samples' DCF (net-new) from the Pipe-Menon fixed-point
