@@ -90,28 +90,6 @@ extern "C" void fr_graph_destroy(fr_graph *gr)
     delete gr;
 }
 
-// Default candidates: unit boxes around the LDS default (16-64 cells) with
-// read-add-write accumulation, 32-cell boxes with the gather kernel, and
-// x-elongated 16/32-cell boxes on the matrix cores.
-static std::vector<fr_tune> default_candidates(int dim)
-{
-    std::vector<fr_tune> c;
-    const int shapes3[][4] = {{4, 4, 2, 2}, {4, 2, 2, 2}, {8, 4, 2, 2}, {4, 4, 2, 4},
-                              {8, 2, 1, 5}, {4, 2, 2, 5}, {8, 1, 2, 5}, {4, 4, 1, 5}};
-    const int shapes2[][4] = {{8, 4, 1, 2}, {4, 4, 1, 2}, {16, 4, 1, 2}, {8, 4, 1, 4},
-                              {8, 2, 1, 5}, {16, 2, 1, 5}, {8, 4, 1, 5}, {4, 4, 1, 5}};
-    const int(*shapes)[4] = dim == 3 ? shapes3 : shapes2;
-    for (int i = 0; i < 8; ++i) {
-        fr_tune t{};
-        t.tile_x = shapes[i][0];
-        t.tile_y = shapes[i][1];
-        t.tile_z = shapes[i][2];
-        t.accumulate = shapes[i][3];
-        c.push_back(t);
-    }
-    return c;
-}
-
 extern "C" fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, const float *lut, int L,
                                  const fr_tune *candidates, int ncand, int reps, fr_tune *best, float *best_ms,
                                  float *cand_ms)
@@ -120,7 +98,7 @@ extern "C" fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, f
     FR_CHECK(ctx, reps >= 1 && reps <= 1000, FR_ERR_INVALID, "reps must be in [1, 1000]");
     std::vector<fr_tune> cand;
     if (candidates && ncand > 0) cand.assign(candidates, candidates + ncand);
-    else cand = default_candidates(s->dim);
+    else cand = fr_tune_structural_candidates(s->dim);
     fr_grids *g = nullptr;
     fr_status st = s->frames > 0 ? fr_grids_create_frames(ctx, N, s->C, s->frames, &g)
                                  : fr_grids_create(ctx, s->dim, N, s->C, &g);

@@ -44,7 +44,11 @@ extern "C" fr_status fr_ctx_create(int device, fr_ctx **out)
     c->device = device;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return FR_ERR_HIP; }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->cu_count = prop.multiProcessorCount;
+        c->arch = prop.gcnArchName;
+        c->arch = c->arch.substr(0, c->arch.find(':'));
+    }
     // this device's static scratch for fr_grid_execute_noalloc
     if (fr_scratch_init(c) != FR_OK) {
         hipStreamDestroy(c->stream);

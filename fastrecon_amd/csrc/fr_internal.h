@@ -17,6 +17,7 @@ struct fr_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     int cu_count = 0;
+    std::string arch;          // gcnArchName up to the first ':' (e.g. gfx950)
     void *scratch = nullptr;   // this device's copy of the static scratch (fr_noalloc.hip)
 };
 
@@ -271,4 +272,5 @@ fr_status fr_plan_build(fr_plan *p, const fr_samples *s, const fr_tune *tune);
 fr_status fr_plan_build_pos(fr_plan *p, const fr_samples *s, const fr_tune *tune);
 // exact LUT-index thresholds of a plan (mfma_pairs, mfma_dacc, mfma_cidx) and,
 // when sel is set, the position-weight selection table (tsel)
+std::vector<fr_tune> fr_tune_structural_candidates(int dim);
 fr_status fr_plan_thresholds(fr_plan *p, bool sel);

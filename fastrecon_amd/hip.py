@@ -38,6 +38,7 @@ EXPORTS = [
     "fr_dcf_pipe_menon",
     "fr_dcf_identity_scale",
     "fr_graph_create", "fr_graph_launch", "fr_graph_destroy", "fr_autotune",
+    "fr_tune_key_of", "fr_tune_cache_lookup", "fr_tune_cache_store", "fr_autotune_search",
     "fr_ldcf_features", "fr_mlp_param_count", "fr_mlp_create", "fr_mlp_destroy", "fr_mlp_set_params",
     "fr_mlp_get_params", "fr_mlp_forward", "fr_mlp_dcf", "fr_ldcf_dcf", "fr_mlp_grad", "fr_mlp_train_step",
     "fr_malloc", "fr_free", "fr_memcpy_d2h", "fr_memcpy_h2d",
@@ -53,6 +54,11 @@ class fr_tune(C.Structure):
                 ("accumulate", C.c_int), ("split_taps", C.c_int),
                 ("exact_order", C.c_int), ("items_per_wave", C.c_int), ("xcd_run", C.c_int),
                 ("stores", C.c_int), ("waves_per_simd", C.c_int), ("tile_writes", C.c_int)]
+
+
+class fr_tune_key(C.Structure):
+    _fields_ = [("arch", C.c_char * 32), ("cus", C.c_int), ("dim", C.c_int), ("N", C.c_uint), ("M", C.c_int64),
+                ("C", C.c_int), ("type", C.c_int), ("frames", C.c_int), ("kwidth", C.c_float), ("L", C.c_int)]
 
 
 class fr_plan_info(C.Structure):
@@ -103,6 +109,12 @@ def load():
         "fr_graph_launch": [_vp],
         "fr_autotune": [_vp, _vp, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_int, C.c_int, C.POINTER(fr_tune),
                         C.POINTER(C.c_float), _vp],
+        "fr_tune_key_of": [_vp, _vp, C.c_uint, C.c_float, C.c_int, C.POINTER(fr_tune_key)],
+        "fr_tune_cache_lookup": [C.c_char_p, C.POINTER(fr_tune_key), C.POINTER(fr_tune), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), C.POINTER(C.c_int)],
+        "fr_tune_cache_store": [C.c_char_p, C.POINTER(fr_tune_key), C.POINTER(fr_tune), C.c_float, C.c_float],
+        "fr_autotune_search": [_vp, _vp, C.c_uint, C.c_float, _vp, C.c_int, C.c_int, C.c_char_p, C.POINTER(fr_tune),
+                               C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int)],
         "fr_grids_destroy": [_vp],
         "fr_grids_download": [_vp, C.c_int, _vp],
         "fr_grids_upload": [_vp, C.c_int, _vp],
@@ -625,7 +637,48 @@ class Graph:
             pass
 
 
-AUTOTUNE_DEFAULT_CANDIDATES = 8     # fr_autotune's built-in list (fr_batch.hip)
+AUTOTUNE_DEFAULT_CANDIDATES = 8     # fr_autotune's built-in list (fr_tune_structural_candidates)
+
+
+def _tune_dict(t):
+    return {k: getattr(t, k) for k, _ in fr_tune._fields_}
+
+
+def autotune_search(ctx, samples, N, kwidth, lut, reps=3, cache_path=None):
+    """Staged autotuning search (fr_autotune_search), with an optional cache
+    file.  Returns (best tune dict, best ms, default ms, timed configurations;
+    0 on a cache hit)."""
+    lut = np.ascontiguousarray(lut, np.float32)
+    best = fr_tune()
+    best_ms, def_ms, timed = C.c_float(), C.c_float(), C.c_int()
+    path = os.fsencode(cache_path) if cache_path else None
+    ctx.check(ctx.L.fr_autotune_search(ctx.h, samples.h, int(N), float(kwidth), _ptr(lut), len(lut), int(reps), path,
+                                       C.byref(best), C.byref(best_ms), C.byref(def_ms), C.byref(timed)),
+              "fr_autotune_search")
+    return _tune_dict(best), best_ms.value, def_ms.value, timed.value
+
+
+def tune_key(ctx, samples, N, kwidth, L):
+    k = fr_tune_key()
+    ctx.check(ctx.L.fr_tune_key_of(ctx.h, samples.h, int(N), float(kwidth), int(L), C.byref(k)), "fr_tune_key_of")
+    return k
+
+
+def tune_cache_lookup(lib, path, key):
+    """Host-only cache read: (tune dict, best ms, default ms) or None on a miss."""
+    t, bm, dm, found = fr_tune(), C.c_float(), C.c_float(), C.c_int()
+    r = lib.fr_tune_cache_lookup(os.fsencode(path), C.byref(key), C.byref(t), C.byref(bm), C.byref(dm),
+                                 C.byref(found))
+    if r != 0:
+        raise HipError(f"fr_tune_cache_lookup failed ({r})")
+    return (_tune_dict(t), bm.value, dm.value) if found.value else None
+
+
+def tune_cache_store(lib, path, key, tune, best_ms, default_ms):
+    t = fr_tune(**{k: int(v) for k, v in tune.items()})
+    r = lib.fr_tune_cache_store(os.fsencode(path), C.byref(key), C.byref(t), float(best_ms), float(default_ms))
+    if r != 0:
+        raise HipError(f"fr_tune_cache_store failed ({r})")
 
 
 def autotune(ctx, samples, N, kwidth, lut, candidates=None, reps=3):
@@ -641,4 +694,4 @@ def autotune(ctx, samples, N, kwidth, lut, candidates=None, reps=3):
     ms = np.zeros(max(n, AUTOTUNE_DEFAULT_CANDIDATES), np.float32)
     ctx.check(ctx.L.fr_autotune(ctx.h, samples.h, int(N), float(kwidth), _ptr(lut), len(lut), cands, n, int(reps),
                                 C.byref(best), C.byref(best_ms), _ptr(ms)), "fr_autotune")
-    return {k: getattr(best, k) for k, _ in fr_tune._fields_}, best_ms.value, ms
+    return _tune_dict(best), best_ms.value, ms

@@ -293,10 +293,10 @@ fr_status fr_graph_create(fr_plan *p, const fr_samples *s, fr_grids *g, fr_fft *
                           unsigned rcz, int apply_shift, const float *deapod_dev, float *out_dev, fr_graph **out);
 fr_status fr_graph_launch(fr_graph *gr);
 void fr_graph_destroy(fr_graph *gr);
-/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: eight
- * defaults -- unit shapes around the default with LDS read-add-write, a
- * 32-cell shape with the gather kernel, and x-elongated 16/32-cell shapes
- * on the matrix cores) on these samples and returns the
+/* Times `reps` gridding passes for each candidate fr_tune (NULL/0: the
+ * eight structural candidates of fr_autotune_search -- 16-cell position-
+ * weight boxes in four aspect ratios, 16-cell matrix-core tap boxes and a
+ * 32-cell LDS read-add-write box) on these samples and returns the
  * fastest in *best (and its time); cand_ms (optional, ncand or 8 entries)
  * gets each candidate's time, -1 for candidates the plan refuses.  The LUT length is
  * not tuned: tap weights are resolved when the plan is built, so L does not
@@ -304,6 +304,45 @@ void fr_graph_destroy(fr_graph *gr);
 fr_status fr_autotune(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, const float *lut, int L,
                       const fr_tune *candidates, int ncand, int reps, fr_tune *best, float *best_ms,
                       float *cand_ms);
+
+/* Key of the autotune cache: the device and the workload's shape (not the
+ * trajectory itself: workloads of one shape share their tune). */
+typedef struct {
+    char arch[32];      /* device architecture, e.g. "gfx950"             */
+    int cus;            /* compute units                                   */
+    int dim;
+    unsigned N;
+    int64_t M;
+    int C;
+    int type;           /* FR_SAMPLES_*                                    */
+    int frames;         /* batched 2D frames (0: one grid)                 */
+    float kwidth;
+    int L;
+} fr_tune_key;
+
+fr_status fr_tune_key_of(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, int L, fr_tune_key *key);
+
+/* The autotune cache file (text, one line per key; host only, no device
+ * work).  lookup: *found = 1 and the stored tune and times on a hit, 0 on a
+ * miss or a missing file.  store: adds or replaces the key's line (written
+ * to path.tmp, then renamed over path). */
+fr_status fr_tune_cache_lookup(const char *path, const fr_tune_key *key, fr_tune *tune, float *best_ms,
+                               float *default_ms, int *found);
+fr_status fr_tune_cache_store(const char *path, const fr_tune_key *key, const fr_tune *tune, float best_ms,
+                              float default_ms);
+
+/* Staged autotuning search on these samples (net-new): the default plan and
+ * the structural candidates; then, around the fastest, the heavy-unit slice
+ * size (split_taps 256 ... 65536) and, for a position-weight plan, joint tile
+ * writes; then that plan's launch-only knobs (items_per_wave 1/2/4, xcd_run
+ * 64 ... 4096, waves_per_simd 3/4, stores 1/2), one at a time.  *best gets
+ * the winner as an explicit tune, *best_ms its time, *default_ms the
+ * default plan's, *timed the number of timed configurations.  With a
+ * cache_path, a cached key returns the stored tune and times with *timed =
+ * 0 and nothing built or timed; a miss searches and stores the result. */
+fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, const float *lut, int L,
+                             int reps, const char *cache_path, fr_tune *best, float *best_ms, float *default_ms,
+                             int *timed);
 
 /* --------------------------------------------- learned density compensation
  * Net-new (the reference has no DCF estimation).  Features per sample, 8

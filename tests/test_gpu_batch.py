@@ -173,3 +173,42 @@ def test_autotune_picks_a_working_candidate(ctx, orc):
     else:
         e = np.linalg.norm(got[:, 0] - ref) / np.linalg.norm(ref)
         assert e <= 1e-5, e
+
+
+def test_autotune_search_and_cache(ctx, orc, tmp_path):
+    """fr_autotune_search on a small 3D kooshball: the staged search times
+    the default and its candidates, its winner grids correctly, and with a
+    cache file a second call returns the stored tune without timing."""
+    import time
+    C, N = 4, 48
+    traj, dcf = syn.kooshball3d(1500, 48)
+    tn, _ = hip.normalize_traj(traj, N)
+    M = tn.shape[0]
+    data = syn.coil_data(C, M, 21)
+    s = hip.Samples(ctx, 3, M, C)
+    s.set_traj(tn)
+    s.set_dcf(dcf)
+    s.set_coils(data)
+    lut, _ = hip.kb_lut(4.0, 2.0, 512)
+    path = str(tmp_path / "tune.txt")
+    t0 = time.perf_counter()
+    best, best_ms, def_ms, timed = hip.autotune_search(ctx, s, N, 4.0, lut, reps=2, cache_path=path)
+    t_search = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    best2, best_ms2, def_ms2, timed2 = hip.autotune_search(ctx, s, N, 4.0, lut, reps=2, cache_path=path)
+    t_hit = time.perf_counter() - t0
+    print(f"[autotune search] {timed} timed in {t_search:.2f} s: best {best} {best_ms:.4f} ms, default "
+          f"{def_ms:.4f} ms; cache hit in {t_hit * 1e3:.2f} ms", flush=True)
+    assert timed >= 9 and def_ms > 0 and 0 < best_ms <= 1.1 * def_ms
+    assert timed2 == 0 and best2 == best and best_ms2 == pytest.approx(best_ms, rel=1e-5)
+    assert def_ms2 == pytest.approx(def_ms, rel=1e-5) and t_hit < 0.1 * t_search
+    # the winner grids correctly
+    p = hip.Plan(ctx, s, N, 4.0, lut, best)
+    g = hip.Grids(ctx, 3, N, C)
+    hip.grid_execute(p, s, g)
+    got = g.download_all()
+    ref = orc.grid_f64(tn, dcf, data, N, 4.0, lut, nthreads=8)
+    e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert e <= 1e-6, e
+    for h in (g, p, s):
+        h.close()

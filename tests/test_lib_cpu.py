@@ -77,3 +77,32 @@ def test_checkpoint_metadata():
     assert isinstance(meta["rows"], int) and meta["rows"] > 0
     assert meta["steps"] > 0 and meta["losses"][-1][1] > 0
     assert z["params"].dtype == np.float32 and z["params"].size == 8 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def test_tune_cache_round_trip(tmp_path):
+    """The autotune cache file through the C ABI (host only): a miss on a
+    missing file, store and hit, a second key, replacement of a key's line,
+    a key differing only in M misses, malformed lines are skipped."""
+    lib = hip.load()
+    path = str(tmp_path / "tune.txt")
+
+    def key(M=13107200, C=32):
+        return hip.fr_tune_key(arch=b"gfx950", cus=256, dim=3, N=256, M=M, C=C, type=0, frames=0, kwidth=4.0, L=512)
+
+    assert hip.tune_cache_lookup(lib, path, key()) is None
+    a = dict(tile_x=4, tile_y=2, tile_z=2, slabs=0, accumulate=6, split_taps=1024, exact_order=0,
+             items_per_wave=2, xcd_run=1024, stores=2, waves_per_simd=4, tile_writes=0)
+    hip.tune_cache_store(lib, path, key(), a, 4.5, 5.25)
+    assert hip.tune_cache_lookup(lib, path, key()) == (a, 4.5, 5.25)
+    b = dict(a, tile_x=8, tile_y=2, tile_z=1, accumulate=5, items_per_wave=0)
+    hip.tune_cache_store(lib, path, key(C=16), b, 9.0, 9.5)
+    a2 = dict(a, xcd_run=64)
+    hip.tune_cache_store(lib, path, key(), a2, 4.25, 5.0)
+    assert hip.tune_cache_lookup(lib, path, key()) == (a2, 4.25, 5.0)
+    assert hip.tune_cache_lookup(lib, path, key(C=16)) == (b, 9.0, 9.5)
+    assert hip.tune_cache_lookup(lib, path, key(M=13107201)) is None
+    lines = open(path).read().splitlines()
+    assert sum("C=32" in l for l in lines) == 1 and sum("C=16" in l for l in lines) == 1
+    with open(path, "a") as f:
+        f.write("garbage line without separators\n" + lines[-1].split(" | ")[0] + " | 1 2 | best_ms=1 default_ms=1\n")
+    assert hip.tune_cache_lookup(lib, path, key(C=16)) == (b, 9.0, 9.5)
