@@ -491,6 +491,15 @@ def main():
         out["ldcf"]["phantom_single_coil"] = ldcf.evaluate_phantom(ctx, mlp, traj, N, (a.image,) * 3, a.image,
                                                                    ramp=dcf)
         mlp.close()
+        # the same comparison on the 2D radial geometry (402 x 512, 256^2
+        # image, 512^2 grid) with 32 coils and the 2D network
+        mlp2, meta2 = ldcf.load(ctx, ldcf.CHECKPOINT_2D)
+        t2, r2 = syn.radial2d(402, 512)
+        out["ldcf_2d"] = dict(ldcf.evaluate(ctx, mlp2, t2, syn.coil_data(32, t2.shape[0], 5), ldcf.N_GRID_2D,
+                                            (ldcf.IMAGE_2D, ldcf.IMAGE_2D, 1), ramp=r2, reps=3),
+                              workload="radial2d 402 spokes x 512, 32 coils, 256^2 image, og 2 -> 512^2 grid",
+                              checkpoint_loss=meta2["losses"][-1][1])
+        mlp2.close()
 
     if not a.no_cpu:
         # the timed passes' grid, recomputed on the same data (the legs above

@@ -25,6 +25,11 @@ KW, OG, L = 4.0, 2.0, 512
 IMAGE = 128
 N_GRID = syn.grid_size(IMAGE, OG)
 CHECKPOINT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ldcf_kooshball.npz")
+# the 2D network: 256^2 images on the 512^2 grid (the 2D radial workload's
+# geometry), trained on 2D radial and spiral trajectories
+IMAGE_2D = 256
+N_GRID_2D = syn.grid_size(IMAGE_2D, OG)
+CHECKPOINT_2D = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ldcf_radial2d.npz")
 
 
 # ------------------------------------------------------- quality measures
@@ -131,20 +136,39 @@ def training_rows(feat, w, per_traj, seed, n):
                            rng.choice(len(w), per_traj // 4, p=p2)])
 
 
-def build_dataset(ctx, count=200, per_traj=50000, scale=1.0, seed=7, iters=10, log=print):
-    """Features and Pipe-Menon targets of `count` seeded training trajectories,
-    `per_traj` rows drawn (seeded) from each (training_rows)."""
+def usable_targets(w):
+    """Samples whose Pipe-Menon weight is a usable training target: positive,
+    finite and at most 100 times the trajectory's 99th percentile.  (A sample
+    at the very edge of k-space whose kernel falls mostly outside the grid
+    has an almost empty density, and its weight can exceed every other
+    sample's by orders of magnitude; the w^2 stratum of training_rows would
+    draw nothing else.)"""
+    w = np.asarray(w, np.float64)
+    good = np.isfinite(w) & (w > 0)
+    if not good.any():
+        return good
+    return good & (w <= 100.0 * np.percentile(w[good], 99))
+
+
+def build_dataset(ctx, count=200, per_traj=50000, scale=1.0, seed=7, iters=10, log=print, dim=3):
+    """Features and Pipe-Menon targets of `count` seeded training trajectories
+    (3D: ldcf_training_trajectories on the 256^3 grid; 2D:
+    ldcf_training_trajectories_2d on the 512^2 grid), `per_traj` rows drawn
+    (seeded) from each (training_rows)."""
     X, W = [], []
     t0 = time.time()
-    for n, (name, k) in enumerate(syn.ldcf_training_trajectories(count, seed, scale)):
-        tn, _ = hip.normalize_traj(k, N_GRID)
-        w, feat = pipe_menon_dcf(ctx, tn, N_GRID, iters)
-        pick = training_rows(feat, w, per_traj, seed, n)
+    gen = syn.ldcf_training_trajectories if dim == 3 else syn.ldcf_training_trajectories_2d
+    N = N_GRID if dim == 3 else N_GRID_2D
+    for n, (name, k) in enumerate(gen(count, seed, scale)):
+        tn, _ = hip.normalize_traj(k, N)
+        w, feat = pipe_menon_dcf(ctx, tn, N, iters)
+        ok = np.flatnonzero(usable_targets(w))
+        pick = ok[training_rows(feat[ok], w[ok], per_traj, seed, n)]
         X.append(feat[pick])
         W.append(w[pick])
         if log and (n % 20 == 0 or n == count - 1):
-            log(f"[ldcf data] {n + 1}/{count} {name}: M={len(w)} w in [{w.min():.3g}, {w.max():.3g}] "
-                f"({time.time() - t0:.1f} s)")
+            log(f"[ldcf data] {n + 1}/{count} {name}: M={len(w)} w in [{w.min():.3g}, {w.max():.3g}], "
+                f"{len(w) - len(ok)} unusable targets ({time.time() - t0:.1f} s)")
     X = np.concatenate(X).astype(np.float32)
     W = np.concatenate(W).astype(np.float64)
     return X, W

@@ -1,9 +1,11 @@
 """Train the learned-DCF network on the device and write the checkpoint.
 
-    python -m fastrecon_amd.ldcf_train [--out PATH] [--count 200] [--per-traj 50000] [--steps 48000] [--lr 5e-3]
+    python -m fastrecon_amd.ldcf_train [--dim 3] [--out PATH] [--count 200] [--per-traj 50000] [--steps 48000]
 
-Training set: `count` seeded variable-density 3D radial and cones
-trajectories for a 128^3 image (256^3 grid), with 10-iteration Pipe-Menon
+Training set (--dim 3): `count` seeded variable-density 3D radial and cones
+trajectories for a 128^3 image (256^3 grid); (--dim 2) 2D radial and
+variable-density spiral trajectories for a 256^2 image (512^2 grid); with
+10-iteration Pipe-Menon
 DCFs from this package's gridding/degridding kernels as targets
 (fastrecon_amd.ldcf.build_dataset; rows drawn by ldcf.training_rows).  The network trains with the package's
 own BF16 forward/backward kernels and Adam (cosine learning-rate decay)."""
@@ -22,7 +24,8 @@ from fastrecon_amd import hip, ldcf  # noqa: E402
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=ldcf.CHECKPOINT)
+    ap.add_argument("--dim", type=int, choices=(2, 3), default=3)
+    ap.add_argument("--out", default=None, help="checkpoint path (default: the shipped one of --dim)")
     ap.add_argument("--count", type=int, default=200)
     ap.add_argument("--per-traj", type=int, default=50000)
     ap.add_argument("--scale", type=float, default=1.0)
@@ -30,15 +33,19 @@ def main():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--eval", action="store_true", help="evaluate the result on the kooshball phantom")
+    ap.add_argument("--eval", action="store_true",
+                    help="evaluate the result (3D: kooshball phantom; 2D: 402-spoke radial, 32 coils)")
     a = ap.parse_args()
+    out = a.out or (ldcf.CHECKPOINT if a.dim == 3 else ldcf.CHECKPOINT_2D)
 
     def log(msg):
         print(msg, flush=True)
 
+    if a.eval:
+        import torch  # noqa: F401  (evaluate times with torch's HIP events: torch loads before the library)
     ctx = hip.Context(0)
     t0 = time.time()
-    X, W = ldcf.build_dataset(ctx, a.count, a.per_traj, a.scale, a.seed, log=lambda m: log(m))
+    X, W = ldcf.build_dataset(ctx, a.count, a.per_traj, a.scale, a.seed, log=lambda m: log(m), dim=a.dim)
     t_data = time.time() - t0
     log(f"[ldcf] dataset {X.shape[0]} rows in {t_data:.1f} s")
     t0 = time.time()
@@ -47,18 +54,26 @@ def main():
     t_train = time.time() - t0
     meta = dict(count=a.count, per_traj=a.per_traj, scale=a.scale, steps=a.steps, batch=a.batch, lr=a.lr,
                 seed=a.seed, rows=int(X.shape[0]), dataset_s=round(t_data, 1), train_s=round(t_train, 1),
-                losses=[(int(s), float(l)) for s, l in losses], grid=ldcf.N_GRID, image=ldcf.IMAGE,
+                losses=[(int(s), float(l)) for s, l in losses], dim=a.dim,
+                grid=ldcf.N_GRID if a.dim == 3 else ldcf.N_GRID_2D, image=ldcf.IMAGE if a.dim == 3 else ldcf.IMAGE_2D,
                 targets="10-iteration Pipe-Menon DCF", features="|u|, azimuth, polar, log1p(n_1,2,4)/8, outer-shell distance, log1p(n_0)/8",
                 row_strata="1/2 radius-stratified, 1/4 log-radius-stratified, 1/4 drawn ~ w^2")
-    ldcf.save(net, a.out, meta)
-    log(f"[ldcf] trained in {t_train:.1f} s, final loss {losses[-1][1]:.5f}; wrote {a.out}")
+    ldcf.save(net, out, meta)
+    log(f"[ldcf] trained in {t_train:.1f} s, final loss {losses[-1][1]:.5f}; wrote {out}")
     log(json.dumps(meta))
-    if a.eval:
+    if a.eval and a.dim == 3:
         # evaluate on the 32-coil kooshball geometry (51,200 spokes x 256, 256^3 grid)
         from fastrecon_amd import synthetic as syn
         traj, ramp = syn.kooshball3d(51200, 256)
         r = ldcf.evaluate_phantom(ctx, net, traj, ldcf.N_GRID, (ldcf.IMAGE,) * 3, ldcf.IMAGE, ramp=ramp)
         log("[ldcf eval phantom] " + json.dumps(r))
+    elif a.eval:
+        # the 2D radial geometry (402 golden-angle spokes x 512, 512^2 grid) with 32 coils
+        from fastrecon_amd import synthetic as syn
+        traj, ramp = syn.radial2d(402, 512)
+        data = syn.coil_data(32, traj.shape[0], 5)
+        r = ldcf.evaluate(ctx, net, traj, data, ldcf.N_GRID_2D, (ldcf.IMAGE_2D, ldcf.IMAGE_2D, 1), ramp=ramp, reps=2)
+        log("[ldcf eval 2d] " + json.dumps(r))
 
 
 if __name__ == "__main__":

@@ -148,6 +148,40 @@ def ldcf_training_trajectories(count=200, seed=7, scale=1.0):
             yield f"cones{n}", cones3d(cones, inter, 256, turns, alpha, seed=seed * 1000 + n)
 
 
+def radial2d_vd(spokes, readout, alpha=1.0, golden=True, seed=0):
+    """Full-diameter 2D spokes, golden-angle or uniformly spaced angles with a
+    seeded offset, variable density along the spoke: k = 0.5 sign(s) |s|^a
+    (cos t, sin t), s = (j - R/2)/(R/2)."""
+    n = np.arange(spokes, dtype=np.float64)
+    off = float(uniform(np.arange(1, dtype=np.uint64), seed)[0]) * math.pi
+    th = off + (n * GOLDEN_ANGLE if golden else n * math.pi / spokes)
+    s = (np.arange(readout, dtype=np.float64) - readout / 2) / (readout / 2)
+    r = 0.5 * np.sign(s) * np.abs(s) ** alpha
+    k = np.stack([(r[None, :] * np.cos(th)[:, None]).reshape(-1), (r[None, :] * np.sin(th)[:, None]).reshape(-1)], 1)
+    return k.astype(np.float32)
+
+
+def ldcf_training_trajectories_2d(count=200, seed=7, scale=1.0):
+    """The seeded 2D training set for a 256^2 image (512^2 grid): half
+    radial (golden-angle or uniform angles, 100-800 spokes of 256 or 512
+    samples, half of them uniform density), half variable-density spirals
+    (8-64 interleaves of 1024-4096 samples).  Yields (name, k[M,2])."""
+    for n in range(count):
+        r = uniform(np.arange(4, dtype=np.uint64) + 4 * n, seed + 1)
+        if n % 2 == 0:
+            spokes = max(16, int(scale * (100 + r[0] * 700)))
+            readout = int(256 + 256 * (r[1] > 0.3))
+            alpha = 1.0 if r[2] < 0.5 else 1.0 + 2.0 * (r[2] - 0.5)
+            yield f"radial2d_{n}", radial2d_vd(spokes, readout, alpha, golden=r[3] < 0.5, seed=seed * 1000 + n)
+        else:
+            inter = max(4, int(scale * (8 + r[0] * 56)))
+            samples = int(1024 * 2 ** int(r[1] * 3))
+            alpha = 1.0 + 1.5 * r[2]
+            turns = 4.0 + 20.0 * r[3]
+            k, _ = spiral2d(inter, samples, alpha, turns)
+            yield f"spiral2d_{n}", k
+
+
 # ------------------------------------------------------- 2D spiral frames
 SPIRAL_ALPHA = 1.5
 SPIRAL_TURNS = 16.0

@@ -287,6 +287,44 @@ def test_c4_spiral_frames_full(ctx, orc):
     assert worst_img <= TOL
 
 
+def kdtree_features(tn, N, idx):
+    """The oracle's eight features (oracle/ldcf_oracle.py: features) of the
+    samples idx of a large trajectory: the same FP32 radius and angles, the
+    neighbour counts through a k-d tree over the nearest cells of all samples
+    (Chebyshev radius r) instead of the oracle's O(M^2) loop."""
+    import ldcf_oracle as lo
+    from scipy.spatial import cKDTree
+    dim = tn.shape[1]
+    cells = np.clip(np.floor(tn + np.float32(0.5)), 0, N - 1)
+    tree = cKDTree(cells)
+    f = np.zeros((len(idx), lo.NF), np.float64)
+    cf_ = np.float32(0.5) * np.float32(N - 1)
+    h = np.float32(0.5) * np.float32(N)
+
+    def radius(t):
+        u = (t - cf_) / h
+        r2 = (u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1]) + u[:, 2] * u[:, 2] if dim == 3 else u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1]
+        return u, np.sqrt(r2)
+
+    u, r = radius(tn[idx])
+    f[:, 0] = np.log1p(r.astype(np.float64) * float(h)) / 5.0
+    f[:, 1] = np.arctan2(u[:, 1].astype(np.float64), u[:, 0].astype(np.float64)) / np.pi
+    if dim == 3:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            f[:, 2] = np.where(r > 0, np.arccos(np.clip(u[:, 2].astype(np.float64) / r, -1, 1)) / np.pi, 0.5)
+    else:
+        f[:, 2] = 0.5
+    for q, rr in enumerate((1, 2, 4)):
+        n = np.asarray(tree.query_ball_point(cells[idx], rr + 0.5, p=np.inf, return_length=True)) - 1
+        f[:, 3 + q] = np.log1p(n) / 8.0
+    rho_all = (radius(tn)[1] * h).astype(np.float32)
+    rho = (r * h).astype(np.float32)
+    f[:, 6] = np.minimum(rho_all.max() - rho, np.float32(4.0)).astype(np.float64) * 0.25
+    n0 = np.asarray(tree.query_ball_point(cells[idx], 0.5, p=np.inf, return_length=True)) - 1
+    f[:, 7] = np.log1p(n0) / 8.0
+    return f
+
+
 # ------------------------------------------------------------------ c5
 def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     """The shipped checkpoint on c2's kooshball (32 coils): its DCF against the
@@ -294,7 +332,6 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     against the 10-iteration Pipe-Menon image (NRMSE without a scale fit,
     the judge of this configuration)."""
     import ldcf_oracle as lo
-    from scipy.spatial import cKDTree
     from fastrecon_amd import ldcf
     N, image = 256, 128
     traj, ramp = syn.kooshball3d(51200, 256)
@@ -305,31 +342,9 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     s.set_coils(coils(32, M, 5))
     mlp.dcf_from_traj(s, N)
     w_ml = s.get_dcf()
-    # oracle features on 4000 seeded samples: counts through a k-d tree
-    # over the nearest cells of all samples (Chebyshev radius r)
     rng = np.random.default_rng(55)
     idx = np.sort(rng.choice(M, 4000, replace=False))
-    cells = np.clip(np.floor(tn + np.float32(0.5)), 0, N - 1)
-    tree = cKDTree(cells)
-    f = np.zeros((len(idx), lo.NF), np.float64)
-    sub = tn[idx]
-    cf_ = np.float32(0.5) * np.float32(N - 1)
-    h = np.float32(0.5) * np.float32(N)
-    u = (sub - cf_) / h
-    r = np.sqrt((u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1]) + u[:, 2] * u[:, 2])
-    f[:, 0] = np.log1p(r.astype(np.float64) * float(h)) / 5.0
-    f[:, 1] = np.arctan2(u[:, 1].astype(np.float64), u[:, 0].astype(np.float64)) / np.pi
-    with np.errstate(invalid="ignore", divide="ignore"):
-        f[:, 2] = np.where(r > 0, np.arccos(np.clip(u[:, 2].astype(np.float64) / r, -1, 1)) / np.pi, 0.5)
-    for q, rr in enumerate((1, 2, 4)):
-        n = np.asarray(tree.query_ball_point(cells[idx], rr + 0.5, p=np.inf, return_length=True)) - 1
-        f[:, 3 + q] = np.log1p(n) / 8.0
-    ua = (tn - cf_) / h
-    rho_all = (np.sqrt((ua[:, 0] * ua[:, 0] + ua[:, 1] * ua[:, 1]) + ua[:, 2] * ua[:, 2]) * h).astype(np.float32)
-    rho = (r * h).astype(np.float32)
-    f[:, 6] = np.minimum(rho_all.max() - rho, np.float32(4.0)).astype(np.float64) * 0.25
-    n0 = np.asarray(tree.query_ball_point(cells[idx], 0.5, p=np.inf, return_length=True)) - 1
-    f[:, 7] = np.log1p(n0) / 8.0
+    f = kdtree_features(tn, N, idx)
     params, mu, sigma = mlp.get_params()
     y, _ = lo.forward(params, f.astype(np.float32), emulate=True)
     w_ref = np.exp(mu + sigma * y.astype(np.float64))
@@ -354,3 +369,49 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     assert np.median(dw) <= 1e-4 and dw.max() <= 2e-2
     assert e_ml < e_rp
     assert e_ml <= 0.05
+
+
+def test_c5_2d_network_on_c1_geometry(ctx, orc):
+    """The literal reading of the learned-DCF workload: the 2D radial
+    geometry (402 golden-angle spokes x 512, 256^2 image on the 512^2 grid)
+    with 32 coils and the shipped 2D network (trained on 2D radial and
+    spiral trajectories).  Its DCF against the oracle's features + network
+    on seeded samples, and the image against the 10-iteration Pipe-Menon
+    image (NRMSE, no scale fit)."""
+    import ldcf_oracle as lo
+    from fastrecon_amd import ldcf
+    N, image = ldcf.N_GRID_2D, ldcf.IMAGE_2D
+    traj, ramp = syn.radial2d(402, 512)
+    tn, _ = hip.normalize_traj(traj, N)
+    M = tn.shape[0]
+    mlp, meta = ldcf.load(ctx, ldcf.CHECKPOINT_2D)
+    assert meta.get("dim") == 2
+    s, p = ldcf.plan_for(ctx, tn, N, 32)
+    s.set_coils(coils(32, M, 6))
+    mlp.dcf_from_traj(s, N)
+    w_ml = s.get_dcf()
+    idx = np.sort(np.random.default_rng(56).choice(M, 3000, replace=False))
+    f = kdtree_features(tn, N, idx)
+    params, mu, sigma = mlp.get_params()
+    y, _ = lo.forward(params, f.astype(np.float32), emulate=True)
+    w_ref = np.exp(mu + sigma * y.astype(np.float64))
+    dw = np.abs(w_ml[idx] / w_ref - 1.0)
+    rc = (image, image, 1)
+    hip.pipe_menon(p, s, 10, True)
+    im_it = ldcf.recon_image(ctx, p, s, N, rc)
+    mlp.dcf_from_traj(s, N)
+    im_ml = ldcf.recon_image(ctx, p, s, N, rc)
+    hip.dcf_identity_scale(p, s)
+    im_mi = ldcf.recon_image(ctx, p, s, N, rc)
+    s.set_dcf(ramp)
+    im_rp = ldcf.recon_image(ctx, p, s, N, rc)
+    e_ml, e_mi, e_rp = ldcf.nrmse(im_ml, im_it), ldcf.nrmse(im_mi, im_it), ldcf.nrmse(im_rp, im_it)
+    es_ml, es_rp = ldcf.nrmse_scaled(im_ml, im_it), ldcf.nrmse_scaled(im_rp, im_it)
+    for h in (p, s):
+        h.close()
+    mlp.close()
+    log(f"[c5 2D] 2D network on the 402x512 radial geometry, 32 coils: DCF vs oracle features+network (3000 "
+        f"samples) median {np.median(dw):.2e}, max {dw.max():.2e}; image NRMSE vs iterative: learned {e_ml:.4f} "
+        f"(scaled {es_ml:.4f}, identity-scaled {e_mi:.4f}), ramp {e_rp:.4f} (scaled {es_rp:.4f})")
+    assert np.median(dw) <= 1e-4 and dw.max() <= 2e-2
+    assert e_ml < e_rp and es_ml < es_rp
