@@ -207,7 +207,7 @@ extern "C" fr_status fr_tune_cache_store(const char *path, const fr_tune_key *ke
     }
     if (keep.empty()) keep.push_back("# fastrecon_amd autotune cache: <key> | <fr_tune fields> | <times>");
     char times[96];
-    snprintf(times, sizeof times, "best_ms=%.6f default_ms=%.6f", (double)best_ms, (double)default_ms);
+    snprintf(times, sizeof times, "best_ms=%.9g default_ms=%.9g", (double)best_ms, (double)default_ms);
     keep.push_back(want + " | " + tune_string(*tune) + " | " + times);
     const std::string tmp = std::string(path) + ".tmp";
     {
