@@ -8,7 +8,9 @@
 //      for position-weight plans, joint tile writes: one plan each;
 //   C. for a position-weight winner, the launch-only knobs on that one plan
 //      (items per wave, XCD run length, waves per SIMD, store kind), one knob
-//      at a time in that order, each kept at its fastest value.
+//      at a time in that order, each kept at its fastest value;
+//   D. the default and the winner again, in turn, three rounds each: the
+//      winner is kept only if its best time beats the default's by 1%.
 // The winner is returned as an explicit fr_tune (a default winner is spelled
 // out from its plan), so a plan built from it reproduces what was timed.
 //
@@ -20,6 +22,7 @@
 // returns the stored tune without building or timing anything.
 #include "fr_internal.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -268,8 +271,14 @@ extern "C" fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsign
         if (!t) def_ms = ms;
         return e;
     };
-    // A: default and structural candidates
+    // A: default (twice: the first pass also brings the clocks up) and the
+    // structural candidates
     st = consider(nullptr);
+    if (!st) {
+        win_ms = -1.f;
+        st = consider(nullptr);
+    }
+    const fr_tune def_tune = win;
     for (const fr_tune &c : fr_tune_structural_candidates(s->dim))
         if (!st) st = consider(&c);
     // B: slice size and (position-weight) joint tile writes around the winner
@@ -318,6 +327,32 @@ extern "C" fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsign
                 *k.plan_field = keep;
             }
             fr_plan_destroy(p);
+        }
+    }
+    // confirmation: the default and the winner timed in turn, three rounds,
+    // best of each; the winner must beat the default by 1% or the default
+    // (spelled out) is returned
+    if (!st && win_ms >= 0.f && def_ms >= 0.f && memcmp(&win, &def_tune, sizeof(fr_tune)) != 0) {
+        fr_plan *pd = nullptr, *pw = nullptr;
+        float md = -1.f, mw = -1.f, m = -1.f;
+        st = T.time_tune(nullptr, lut, L, N, kWidth, &md, nullptr, &pd);
+        if (!st) st = T.time_tune(&win, lut, L, N, kWidth, &mw, nullptr, &pw);
+        for (int r = 0; r < 2 && !st && pd && pw; ++r) {
+            st = T.time(pd, &m);
+            if (!st) md = std::min(md, m);
+            if (!st) st = T.time(pw, &m);
+            if (!st) mw = std::min(mw, m);
+        }
+        if (pd) fr_plan_destroy(pd);
+        if (pw) fr_plan_destroy(pw);
+        if (!st && md >= 0.f && mw >= 0.f) {
+            def_ms = md;
+            if (mw < 0.99f * md) {
+                win_ms = mw;
+            } else {
+                win = def_tune;
+                win_ms = md;
+            }
         }
     }
     hipEventDestroy(T.e0);

@@ -335,9 +335,11 @@ fr_status fr_tune_cache_store(const char *path, const fr_tune_key *key, const fr
  * the structural candidates; then, around the fastest, the heavy-unit slice
  * size (split_taps 256 ... 65536) and, for a position-weight plan, joint tile
  * writes; then that plan's launch-only knobs (items_per_wave 1/2/4, xcd_run
- * 64 ... 4096, waves_per_simd 3/4, stores 1/2), one at a time.  *best gets
- * the winner as an explicit tune, *best_ms its time, *default_ms the
- * default plan's, *timed the number of timed configurations.  With a
+ * 64 ... 4096, waves_per_simd 3/4, stores 1/2), one at a time; finally the
+ * default and the winner in turn, three rounds each, and the winner is kept
+ * only if it beats the default by 1%.  *best gets the winner as an explicit
+ * tune (the default's settings spelled out when it wins), *best_ms its time,
+ * *default_ms the default plan's, *timed the number of timed passes.  With a
  * cache_path, a cached key returns the stored tune and times with *timed =
  * 0 and nothing built or timed; a miss searches and stores the result. */
 fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsigned N, float kWidth, const float *lut, int L,
