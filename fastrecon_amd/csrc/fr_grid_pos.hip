@@ -15,7 +15,9 @@
 //   A[i][k] = LUT weight of record k's sample at cell i, 0 where the sample
 //             has no tap there: computed here, lane (i, k) evaluating the
 //             reference tap test and LUT index for its own cell;
-//   B[k][j] = dcf * sample component j of record k (GridLut.cpp:109).
+//   B[k][j] = dcf * sample component j of record k (GridLut.cpp:109); the
+//             DCF travels in the .w of the record's position entry, which
+//             k_pos_dcf refreshes from the samples before every pass.
 //
 // Tap test and LUT index (GridLut.cpp:112-136) without the reference's
 // correctly rounded square root and division: the squared distance d2 is
@@ -100,16 +102,18 @@ __device__ __forceinline__ void load_ids(uint32_t (&id)[4], const uint32_t *__re
     for (int q = 0; q < 4; ++q) id[q] = ld_off(recs, (rb + 4 * q + k) * (uint32_t)sizeof(uint32_t));
 }
 
+// The position entry carries the sample's DCF in .w (refreshed from the
+// samples before every pass), so a record costs one 16-byte gather for
+// position and weight.
 template <int NCB, bool PAIR, typename TS>
 __device__ __forceinline__ void issue_batch(batch_st &S, const uint32_t (&id)[4], const float4 *__restrict__ pos,
-                                            const float *__restrict__ dcf, const TS *__restrict__ data, uint32_t C,
-                                            int ca, int cbb)
+                                            const TS *__restrict__ data, uint32_t C, int ca, int cbb)
 {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const uint32_t s = id[q];
         S.p[q] = ld_off(pos, s * (uint32_t)sizeof(float4));
-        S.w[q] = ld_off(dcf, s * (uint32_t)sizeof(float));
+        S.w[q] = S.p[q].w;
         const TS *row = data + (uint64_t)s * C;
         if constexpr (NCB == 4 && PAIR) {
             const float4 v = ld_pair(row, ca);
@@ -214,7 +218,7 @@ __device__ __forceinline__ void lane_cell(const item_box &b, int j, float &fx, f
 template <int NCB, bool PAIR, bool FULL, typename TS>
 __device__ __forceinline__ void grid_records(f4 (&acc)[NCB], uint32_t r, uint32_t r1,
                                              const uint32_t *__restrict__ recs, const float4 *__restrict__ pos,
-                                             const float *__restrict__ dcf, const TS *__restrict__ data, uint32_t C,
+                                             const TS *__restrict__ data, uint32_t C,
                                              int ca, int cbb, const float4 *stbl, float fx, float fy, float fz,
                                              float dacc, float cidx, float lmax, bool v0ok, bool v1ok, int j, int k)
 {
@@ -222,16 +226,16 @@ __device__ __forceinline__ void grid_records(f4 (&acc)[NCB], uint32_t r, uint32_
     batch_st S0, S1;
     load_ids(id0, recs, r, k);
     load_ids(id1, recs, r + RB, k);
-    issue_batch<NCB, PAIR>(S0, id0, pos, dcf, data, C, ca, cbb);
+    issue_batch<NCB, PAIR>(S0, id0, pos, data, C, ca, cbb);
     while (true) {
         load_ids(id0, recs, r + 2 * RB, k);
-        issue_batch<NCB, PAIR>(S1, id1, pos, dcf, data, C, ca, cbb);
+        issue_batch<NCB, PAIR>(S1, id1, pos, data, C, ca, cbb);
         __builtin_amdgcn_sched_barrier(0);
         apply_batch<NCB, FULL>(acc, S0, stbl, fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j);
         r += RB;
         if (r >= r1) break;
         load_ids(id1, recs, r + 2 * RB, k);
-        issue_batch<NCB, PAIR>(S0, id0, pos, dcf, data, C, ca, cbb);
+        issue_batch<NCB, PAIR>(S0, id0, pos, data, C, ca, cbb);
         __builtin_amdgcn_sched_barrier(0);
         apply_batch<NCB, FULL>(acc, S1, stbl, fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j);
         r += RB;
@@ -256,7 +260,7 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     const fr_item *__restrict__ items, int64_t nitems, const fr_tile *__restrict__ tiles, int64_t ntiles,
     int lbx, int lby, int lbz, const uint32_t *__restrict__ recs, const float4 *__restrict__ pos,
     const float4 *__restrict__ tbl, int L, float dacc, float cidx, const TS *__restrict__ data,
-    const float *__restrict__ dcf, int C, int N, size_t cells, float2 *__restrict__ out, float2 *__restrict__ partial,
+    int C, int N, size_t cells, float2 *__restrict__ out, float2 *__restrict__ partial,
     int64_t nslots, int cells_max, int64_t nfront, int64_t front_wg, int ipw, int64_t xr, int nt)
 {
     extern __shared__ float4 smem_pos[];
@@ -303,7 +307,7 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
             float fx, fy, fz;
             lane_cell(unpack_box(it.t0, it.pad), j, fx, fy, fz);
             if (it.r0 < it.r1)
-                grid_records<NCB, PAIR, FULL>(acc, it.r0, it.r1, recs, pos, dcf, data, (uint32_t)C, ca, cbb, stbl, fx,
+                grid_records<NCB, PAIR, FULL>(acc, it.r0, it.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl, fx,
                                               fy, fz, dacc, cidx, lmax, v0ok, v1ok, j, k);
             // register r holds cell q = 4 k + r of the unit, component NCB j + cb
 #pragma unroll
@@ -434,16 +438,16 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     batch_st S0, S1;
     load_ids(id0, recs, r, k);
     load_ids(id1, recs, r + RB, k);
-    issue_batch<NCB, PAIR>(S0, id0, pos, dcf, data, (uint32_t)C, ca, cbb);
+    issue_batch<NCB, PAIR>(S0, id0, pos, data, (uint32_t)C, ca, cbb);
     while (true) {
         load_ids(id0, recs, r + 2 * RB, k);
-        issue_batch<NCB, PAIR>(S1, id1, pos, dcf, data, (uint32_t)C, ca, cbb);
+        issue_batch<NCB, PAIR>(S1, id1, pos, data, (uint32_t)C, ca, cbb);
         __builtin_amdgcn_sched_barrier(0);
         apply_batch<NCB, FULL>(acc, S0, stbl, fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j);
         r += RB;
         if (r == cur.r1 && !next_item()) break;
         load_ids(id1, recs, r + 2 * RB, k);
-        issue_batch<NCB, PAIR>(S0, id0, pos, dcf, data, (uint32_t)C, ca, cbb);
+        issue_batch<NCB, PAIR>(S0, id0, pos, data, (uint32_t)C, ca, cbb);
         __builtin_amdgcn_sched_barrier(0);
         apply_batch<NCB, FULL>(acc, S1, stbl, fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j);
         r += RB;
@@ -474,7 +478,7 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
               lbz = __builtin_ctz((unsigned)p->geom.bz);
     hipLaunchKernelGGL(kern, dim3((unsigned)W, chunks), dim3(64 * PW), lds, ctx->stream, p->items, p->nitems,
                        p->tiles, p->ntiles, lbx, lby, lbz, p->recs, p->pos, p->tsel, p->L, p->mfma_dacc, p->mfma_cidx,
-                       (const TS *)s->data, s->dcf, s->C, (int)p->N, cells, out, partial, p->nslots,
+                       (const TS *)s->data, s->C, (int)p->N, cells, out, partial, p->nslots,
                        p->geom.cells_max, p->nitems_front, front_wg, ipw, xr, p->pos_stores == 1 ? 1 : 0);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
@@ -490,6 +494,13 @@ fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out, float2 *parti
     return launch<4, false, false, TS, OCC>(p, s, out, partial);
 }
 
+// pos[i].w <- the samples' DCF (the null sample keeps 0)
+__global__ void k_pos_dcf(float4 *__restrict__ pos, const float *__restrict__ dcf, int64_t M)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) pos[i].w = dcf[i];
+}
+
 template <typename TS>
 fr_status launch_occ(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
 {
@@ -502,6 +513,13 @@ fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float
 {
     FR_CHECK(p->ctx, p->recs && p->pos && p->tsel && p->geom.cells_max <= 16 && p->L <= FR_POS_MAX_L,
              FR_ERR_STATE, "position-weight gridding needs a plan built for it (accumulate 6)");
+    // the samples' DCF into the positions' .w, on every pass: the plan may
+    // serve samples handles with different DCFs, and a captured graph
+    // replays this copy with the pass
+    if (p->M > 0)
+        hipLaunchKernelGGL(k_pos_dcf, dim3((unsigned)((p->M + 255) / 256)), dim3(256), 0, p->ctx->stream, p->pos,
+                           s->dcf, p->M);
+    FR_HIP(p->ctx, hipGetLastError());
     return s->type == FR_SAMPLES_C64 ? launch_occ<float2>(p, s, out, partial)
                                      : launch_occ<__half2>(p, s, out, partial);
 }

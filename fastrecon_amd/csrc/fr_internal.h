@@ -179,8 +179,9 @@ struct fr_plan {
     // position-weight mode (FR_ACC_POS): no taps.  hdrs is unused; recs holds
     // each unit's sample indices in ascending order, padded with the index M
     // to a multiple of FR_POS_BATCH (every unit starts at such a multiple),
-    // then 64 more entries M; pos[M + 1] the samples' grid positions (x, y,
-    // z, 0), entry M far outside the grid; tsel[L] the LUT-index selection
+    // then 64 more entries M; pos[M + 1] the samples' grid positions and
+    // DCF (x, y, z, w; w copied from the samples before every pass), entry
+    // M far outside the grid with w = 0; tsel[L] the LUT-index selection
     // table (B[m+1], LUT[m], LUT[m+1], 0) for the estimate m
     // (fr_plan_pos.hip).  Items carry their unit's box: t0 = ox | oy << 16,
     // pad = oz | (ux | uy << 5 | uz << 10) << 16.

@@ -65,7 +65,7 @@ __global__ void k_positions(const float *__restrict__ traj, int64_t M, int dim, 
         z = (float)min(max((int)traj[2 * M + i], 0), frames - 1);
     else if (dim == 3)
         z = traj[2 * M + i];
-    pos[i] = make_float4(traj[i], traj[M + i], z, 0.f);
+    pos[i] = make_float4(traj[i], traj[M + i], z, 0.f);     // .w: the DCF, set before each pass
 }
 
 // Does the sample of box b have an accepted tap in [lo, hi] (inclusive)?
