@@ -261,13 +261,14 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     int lbx, int lby, int lbz, const uint32_t *__restrict__ recs, const float4 *__restrict__ pos,
     const float4 *__restrict__ tbl, int L, float dacc, float cidx, const TS *__restrict__ data,
     int C, int N, size_t cells, float2 *__restrict__ out, float2 *__restrict__ partial,
-    int64_t nslots, int cells_max, int64_t nfront, int64_t front_wg, int ipw, int64_t xr, int nt)
+    int64_t nslots, int cells_max, int64_t nfront, int64_t front_wg, int ipw, int64_t xr, int nt, int tw,
+    int osm_floats)
 {
     extern __shared__ float4 smem_pos[];
     float4 *stbl = smem_pos;                                         // L entries
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    float *osm = reinterpret_cast<float *>(smem_pos + L) + wv * (16 * NCB * CS);
+    float *osm = reinterpret_cast<float *>(smem_pos + L) + wv * osm_floats;
     for (int q = threadIdx.x; q < L; q += 64 * PW) stbl[q] = tbl[q];
     __syncthreads();
 
@@ -293,7 +294,7 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     const int64_t rr = bid - front_wg, gg = rr / 8;
     const int64_t grp = (gg / xr) * (8 * xr) + (bid % 8) * xr + gg % xr;
 
-    if (bid >= front_wg && ntiles > 0) {
+    if (bid >= front_wg && ntiles > 0 && tw == PW) {
         // ---------------------------------------------------------- tile
         if (grp >= ntiles) return;              // the whole workgroup
         const fr_tile T = tiles[grp];
@@ -347,16 +348,6 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
         return;
     }
 
-    int64_t i, iend;
-    if (bid < front_wg) {
-        i = bid * PW + wv;
-        iend = min(i + 1, nfront);
-    } else {
-        i = nfront + (grp * PW + wv) * ipw;
-        iend = min(i + ipw, nitems);
-    }
-    if (i >= iend) return;
-
     // Write the accumulators of item it (unit box b) through an LDS
     // transpose: register r holds cell 4 k + r, component NCB j + cb (coil
     // c: re 2c, im 2c + 1); consecutive lanes then store consecutive cells
@@ -402,6 +393,115 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
     };
+
+    // ----------------------------------------------------------- pairs
+    // One wave grids the two x-adjacent units of a pair one after the other
+    // (the first unit's sums held in registers) and writes them out
+    // together: rows of 2 bx cells (64 bytes for 4-cell-wide units) instead
+    // of bx.  Components go through the LDS transpose in steps of 16 coils
+    // (8 when a chunk has 8): region cell x + 2 bx (y + by z) of the pair's
+    // 2 bx x by x bz box, unit A at x < bx, unit B at x >= bx.
+    if (bid >= front_wg && ntiles > 0) {
+        const int64_t pidx = grp * PW + wv;
+        if (pidx >= ntiles) return;
+        const fr_tile T = tiles[pidx];
+        const int32_t ia = T.item[0], ib = T.item[1];
+        if (ia < 0 && ib < 0) return;
+        // a unit alone (its partner is a front unit): the single write-out
+        if (ia < 0 || ib < 0) {
+            const fr_item it = items[ia >= 0 ? ia : ib];
+            const item_box b = unpack_box(it.t0, it.pad);
+            float fx, fy, fz;
+            lane_cell(b, j, fx, fy, fz);
+            if (it.r0 < it.r1)
+                grid_records<NCB, PAIR, FULL>(acc, it.r0, it.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl, fx, fy,
+                                              fz, dacc, cidx, lmax, v0ok, v1ok, j, k);
+            write_out(it, b);
+            return;
+        }
+        constexpr int RS = 33;                        // LDS row: 32 region cells + 1
+        const int bxs = 1 << lbx, lb2 = lbx + 1;
+        constexpr int steps = NCB == 4 ? 2 : 1;
+        constexpr int ncs = NCB == 1 ? 8 : 16;        // coils per step
+        constexpr int c1 = NCB < 2 ? NCB : 2;         // component slices of the first step
+        // lane (j, k): register r holds cell q = 4 k + r of a unit; unit A's
+        // cells go to region x < bx, B's to x >= bx
+        auto stage = [&](f4 a, int cb, int xoff) {
+            const int comp = NCB == 1 ? j : 2 * j + (cb & 1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = 4 * k + r;
+                osm[comp * RS + (q & (bxs - 1)) + ((q >> lbx) << lb2) + xoff] = a[r];
+            }
+        };
+        const fr_item ita = items[ia], itb = items[ib];
+        const item_box ba = unpack_box(ita.t0, ita.pad), bb = unpack_box(itb.t0, itb.pad);
+        {
+            float fx, fy, fz;
+            lane_cell(ba, j, fx, fy, fz);
+            if (ita.r0 < ita.r1)
+                grid_records<NCB, PAIR, FULL>(acc, ita.r0, ita.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl, fx,
+                                              fy, fz, dacc, cidx, lmax, v0ok, v1ok, j, k);
+        }
+        // A's first-step component slices into the transpose buffer now; for
+        // NCB 4 its others (coils 2j + 1) wait in registers
+        wave_sync();
+#pragma unroll
+        for (int cb = 0; cb < c1; ++cb) stage(acc[cb], cb, 0);
+        f4 hold0 = acc[0], hold1 = acc[0];
+        if constexpr (NCB == 4) {
+            hold0 = acc[2];
+            hold1 = acc[3];
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+        {
+            float fx, fy, fz;
+            lane_cell(bb, j, fx, fy, fz);
+            if (itb.r0 < itb.r1)
+                grid_records<NCB, PAIR, FULL>(acc, itb.r0, itb.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl, fx,
+                                              fy, fz, dacc, cidx, lmax, v0ok, v1ok, j, k);
+        }
+        const int cell = lane & 31;
+        const int x = cell & ((2 << lbx) - 1), row = cell >> lb2;
+        const size_t go = ((size_t)(ba.oz + (row >> lby)) * N + (ba.oy + (row & ((1 << lby) - 1)))) * N + (ba.ox + x);
+#pragma unroll
+        for (int h = 0; h < steps; ++h) {
+            if (h == 1) {
+                wave_sync();
+                stage(hold0, 2, 0);
+                stage(hold1, 3, 0);
+            }
+#pragma unroll
+            for (int cb = 2 * h; cb < (2 * h + 2 < NCB ? 2 * h + 2 : NCB); ++cb) stage(acc[cb], cb, bxs);
+            wave_sync();
+#pragma unroll
+            for (int t = 0; t < ncs / 2; ++t) {
+                const int cs = (lane >> 5) + 2 * t;
+                const int cl = NCB == 4 ? 2 * cs + h : cs;
+                if (cl < nc) {
+                    const float2 v = make_float2(osm[(2 * cs) * RS + cell], osm[(2 * cs + 1) * RS + cell]);
+                    float2 *d = out + (size_t)(c0 + cl) * cells + go;
+                    if (nt)
+                        __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v),
+                                                    reinterpret_cast<unsigned long long *>(d));
+                    else
+                        *d = v;
+                }
+            }
+        }
+        return;
+    }
+
+    int64_t i, iend;
+    if (bid < front_wg) {
+        i = bid * PW + wv;
+        iend = min(i + 1, nfront);
+    } else {
+        i = nfront + (grp * PW + wv) * ipw;
+        iend = min(i + ipw, nitems);
+    }
+    if (i >= iend) return;
 
     // current item; empty items (units without records) write zeros
     fr_item cur = items[i];
@@ -466,12 +566,19 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
     const int64_t front_wg = (p->nitems_front + PW - 1) / PW;
     const int64_t light = p->nitems - p->nitems_front;
     // light work: tiles (one per workgroup) or runs of ipw items per wave
-    const int64_t groups = p->ntiles > 0 ? p->ntiles : (light + per_wg - 1) / per_wg;
-    const int64_t xr = std::max<int64_t>(1, p->xcd_run / (p->ntiles > 0 ? PW : per_wg));
+    // light work: tiles of four (one per workgroup), pairs (PW per
+    // workgroup) or runs of ipw items per wave
+    const bool fours = p->ntiles > 0 && p->tile_units == PW;
+    const int64_t groups = p->ntiles > 0 ? (fours ? p->ntiles : (p->ntiles + PW - 1) / PW)
+                                         : (light + per_wg - 1) / per_wg;
+    const int64_t xr = std::max<int64_t>(1, p->xcd_run / (p->ntiles > 0 ? (fours ? PW : 2 * PW) : per_wg));
     const int64_t span = 8 * xr;
     const int64_t W = front_wg + span * ((groups + span - 1) / span);
     FR_CHECK(ctx, W < (int64_t(1) << 31), FR_ERR_UNSUPPORTED, "gridding launch of %lld workgroups", (long long)W);
-    const size_t lds = sizeof(float4) * p->L + sizeof(float) * PW * 16 * NCB * CS;
+    // per wave: the write-out transpose (16 cells x 16 NCB components, or a
+    // pair's 32 cells x 32 components)
+    const int osm_floats = std::max(16 * NCB * CS, 32 * 33);
+    const size_t lds = sizeof(float4) * p->L + sizeof(float) * PW * osm_floats;
     auto kern = k_grid_pos<NCB, PAIR, FULL, TS, OCC>;
     FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int lbx = __builtin_ctz((unsigned)p->geom.bx), lby = __builtin_ctz((unsigned)p->geom.by),
@@ -479,7 +586,8 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
     hipLaunchKernelGGL(kern, dim3((unsigned)W, chunks), dim3(64 * PW), lds, ctx->stream, p->items, p->nitems,
                        p->tiles, p->ntiles, lbx, lby, lbz, p->recs, p->pos, p->tsel, p->L, p->mfma_dacc, p->mfma_cidx,
                        (const TS *)s->data, s->C, (int)p->N, cells, out, partial, p->nslots,
-                       p->geom.cells_max, p->nitems_front, front_wg, ipw, xr, p->pos_stores == 1 ? 1 : 0);
+                       p->geom.cells_max, p->nitems_front, front_wg, ipw, xr, p->pos_stores == 1 ? 1 : 0,
+                       p->tile_units, osm_floats);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }

@@ -114,11 +114,12 @@ struct fr_item {
     uint32_t pad;
 };
 
-// Position-weight mode: a tile is FR_POS_WAVES x-adjacent units gridded by
-// one workgroup (one wave each) and written out together in whole rows;
-// item[w] is the item of unit w, or -1 when that unit is written elsewhere
-// (front units) -- its cells are then left alone.  The tile's box: box0 =
-// ox | oy << 16, box1 = oz.
+// Position-weight mode: a tile is tile_units (4 or 2) x-adjacent units
+// written out together: four gridded by one workgroup (one wave each) and
+// stored in whole rows, or a pair gridded by one wave and stored in rows of
+// two units; item[w] is the item of unit w, or -1 when that unit is written
+// elsewhere (front units) -- its cells are then left alone.  The tile's
+// box: box0 = ox | oy << 16, box1 = oz.
 struct fr_tile {
     int32_t item[4];
     uint32_t box0, box1;
@@ -195,6 +196,7 @@ struct fr_plan {
     int64_t nrecs_padded = 0;
     fr_tile *tiles = nullptr;          // position-weight mode: light units by tile, launch order
     int64_t ntiles = 0;                // 0: light items are written unit by unit
+    int tile_units = 0;                // units per tile: 4 (workgroup tiles) or 2 (wave pairs)
     double build_ms = 0.0;
 };
 

@@ -18,9 +18,11 @@
 //   5. items: each box's records cut into slices of at most S records
 //      (a multiple of 16); boxes with several slices write partial sums to
 //      scratch slots, combined afterwards in slice order (k_combine);
-//   6. tiles: the light boxes grouped by FR_POS_WAVES x-adjacent boxes, in
-//      Morton order of the tiles; a workgroup grids one tile and writes it
-//      out in whole rows.
+//   6. tiles: the light boxes grouped by TW x-adjacent boxes, in Morton
+//      order of the tiles: pairs (TW = 2, the default), which a wave grids
+//      one after the other and writes out in rows of two units (64 bytes for
+//      4-cell-wide units), or fours (TW = 4), which a workgroup grids and
+//      writes out in whole rows.
 #include "fr_internal.h"
 #include "fr_taps.h"
 
@@ -264,8 +266,8 @@ __global__ void k_items(int64_t nu, uint32_t S, const uint64_t *__restrict__ P, 
     if (nit > 1) heavy[HI[u]] = fr_heavy{(uint32_t)u, nit, (uint32_t)HS[u], 0u};
 }
 
-// Tiles: FR_POS_WAVES x-adjacent units.  Sort key: Morton order of the
-// tile coordinates (frame-major for batched frames).
+// Tiles: TW x-adjacent units.  Sort key: Morton order of the tile
+// coordinates (frame-major for batched frames).
 __global__ void k_tile_keys(int64_t ntiles, int ntx, int nty, int frames, uint64_t *__restrict__ key,
                             uint32_t *__restrict__ val)
 {
@@ -278,7 +280,7 @@ __global__ void k_tile_keys(int64_t ntiles, int ntx, int nty, int frames, uint64
 }
 
 // tiles in launch order: box and no items yet; tpos = inverse permutation
-__global__ void k_tile_init(const uint32_t *__restrict__ torder, int64_t ntiles, int ntx, int nty, pgeom g,
+__global__ void k_tile_init(const uint32_t *__restrict__ torder, int64_t ntiles, int ntx, int nty, pgeom g, int tw,
                             fr_tile *__restrict__ tiles, uint32_t *__restrict__ tpos)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,11 +288,11 @@ __global__ void k_tile_init(const uint32_t *__restrict__ torder, int64_t ntiles,
     const uint32_t t = torder[q];
     tpos[t] = (uint32_t)q;
     const uint32_t X = t % ntx, Y = (t / ntx) % nty, Z = t / (ntx * nty);
-    tiles[q] = fr_tile{{-1, -1, -1, -1}, (X * FR_POS_WAVES * g.bx) | ((Y * g.by) << 16), Z * g.bz};
+    tiles[q] = fr_tile{{-1, -1, -1, -1}, (X * tw * g.bx) | ((Y * g.by) << 16), Z * g.bz};
 }
 
 // every light unit's (single) item into its tile slot
-__global__ void k_tile_fill(const uint32_t *__restrict__ order, int64_t nu, int64_t nfront_units, pgeom g, int ntx,
+__global__ void k_tile_fill(const uint32_t *__restrict__ order, int64_t nu, int64_t nfront_units, pgeom g, int ntx, int tw,
                             const uint32_t *__restrict__ tpos, const uint64_t *__restrict__ I,
                             fr_tile *__restrict__ tiles)
 {
@@ -299,8 +301,8 @@ __global__ void k_tile_fill(const uint32_t *__restrict__ order, int64_t nu, int6
     const int64_t b = order[u];
     const uint32_t X = (uint32_t)(b % g.nbx), Y = (uint32_t)((b / g.nbx) % g.nby),
                    Z = (uint32_t)(b / ((int64_t)g.nbx * g.nby));
-    const uint32_t t = (Z * g.nby + Y) * ntx + X / FR_POS_WAVES;
-    tiles[tpos[t]].item[X % FR_POS_WAVES] = (int32_t)I[u];
+    const uint32_t t = (Z * g.nby + Y) * ntx + X / tw;
+    tiles[tpos[t]].item[X % tw] = (int32_t)I[u];
 }
 
 template <typename In>
@@ -361,8 +363,8 @@ fr_status fr_plan_build_pos(fr_plan *p, const fr_samples *s, const fr_tune *tune
     p->geom = geom;
     p->accumulate = FR_ACC_POS;
     p->exact_order = 0;
-    // defaults measured on the 32-coil 256^3 kooshball (5.20 ms; 5.57 with
-    // tile writes, 5.97 with streaming stores at three waves per SIMD)
+    // defaults measured on the 32-coil 256^3 kooshball (4.78 ms with pairs;
+    // 4.95 unit by unit, 5.41 with tiles of four; streaming stores slower)
     p->pos_ipw = 1;
     p->xcd_run = 256;
     p->pos_stores = 2;
@@ -372,7 +374,7 @@ fr_status fr_plan_build_pos(fr_plan *p, const fr_samples *s, const fr_tune *tune
     if (tune && tune->stores > 0) p->pos_stores = tune->stores;
     if (tune && tune->waves_per_simd > 0) p->pos_waves = tune->waves_per_simd;
     const int tile_writes = tune ? tune->tile_writes : 0;
-    FR_CHECK(ctx, tile_writes >= 0 && tile_writes <= 2, FR_ERR_INVALID, "tile_writes %d out of range", tile_writes);
+    FR_CHECK(ctx, tile_writes >= 0 && tile_writes <= 3, FR_ERR_INVALID, "tile_writes %d out of range", tile_writes);
     FR_CHECK(ctx, (p->pos_stores == 1 || p->pos_stores == 2) && (p->pos_waves == 3 || p->pos_waves == 4),
              FR_ERR_INVALID, "stores %d / waves_per_simd %d out of range", p->pos_stores, p->pos_waves);
     FR_CHECK(ctx, p->pos_ipw <= 64 && p->xcd_run <= (1 << 20), FR_ERR_INVALID, "items_per_wave %d / xcd_run %d "
@@ -511,15 +513,18 @@ fr_status fr_plan_build_pos(fr_plan *p, const fr_samples *s, const fr_tune *tune
                        c4_r.p, nit_r.p, p->units, p->items, p->heavy);
     FR_HIP(ctx, hipGetLastError());
 
-    // 6. tiles of FR_POS_WAVES x-adjacent whole units (unit box of 16 cells
-    // with power-of-two sides; the grid divides into whole tiles), so a
-    // workgroup writes rows of FR_POS_WAVES * bx cells (128 bytes)
+    // 6. tiles of TW x-adjacent whole units (unit box of 16 cells with
+    // power-of-two sides; the grid divides into whole tiles): pairs by
+    // default (tile_writes 0 or 3), rows of 2 bx cells; fours with
+    // tile_writes 1, rows of 4 bx cells; none with tile_writes 2
     const bool pow2 = (geom.bx & (geom.bx - 1)) == 0 && (geom.by & (geom.by - 1)) == 0 &&
                       (geom.bz & (geom.bz - 1)) == 0;
+    const int TW = tile_writes == 1 ? FR_POS_WAVES : (tile_writes == 2 ? 0 : 2);
     p->ntiles = 0;
-    if (tile_writes == 1 && pow2 && geom.cells_max == 16 && N % (FR_POS_WAVES * geom.bx) == 0 && N % geom.by == 0 &&
+    p->tile_units = 0;
+    if (TW > 0 && pow2 && geom.cells_max == 16 && N % (TW * geom.bx) == 0 && N % geom.by == 0 &&
         (flat || N % geom.bz == 0) && nu - (int64_t)tot.nfront_units > 0) {
-        const int ntx = g.nbx / FR_POS_WAVES;
+        const int ntx = g.nbx / TW;
         const int64_t ntiles = (int64_t)ntx * g.nby * g.nbz;
         dbuf<uint64_t> tk, tk2;
         dbuf<uint32_t> tv, torder, tpos;
@@ -538,13 +543,14 @@ fr_status fr_plan_build_pos(fr_plan *p, const fr_samples *s, const fr_tune *tune
         FR_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(stmp.p, need, tk.p, tk2.p, tv.p, torder.p, (int)ntiles, 0, 64,
                                                        st));
         hipLaunchKernelGGL(k_tile_init, dim3(nblk(ntiles, 256)), dim3(256), 0, st, torder.p, ntiles, ntx, g.nby, g,
-                           p->tiles, tpos.p);
+                           TW, p->tiles, tpos.p);
         const int64_t nlight = nu - (int64_t)tot.nfront_units;
         hipLaunchKernelGGL(k_tile_fill, dim3(nblk(nlight, 256)), dim3(256), 0, st, order.p, nu,
-                           (int64_t)tot.nfront_units, g, ntx, tpos.p, I.p, p->tiles);
+                           (int64_t)tot.nfront_units, g, ntx, TW, tpos.p, I.p, p->tiles);
         FR_HIP(ctx, hipGetLastError());
         FR_HIP(ctx, hipStreamSynchronize(st));
         p->ntiles = ntiles;
+        p->tile_units = TW;
     }
     FR_HIP(ctx, hipStreamSynchronize(st));
     p->max_unit_taps = 0;

@@ -5,7 +5,8 @@
 //   A. the default plan and the structural candidates (unit box and
 //      accumulation mode, fr_tune_structural_candidates), one plan each;
 //   B. for the fastest so far, the heavy-unit slice size (split_taps) and,
-//      for position-weight plans, joint tile writes: one plan each;
+//      for position-weight plans, the write-out grouping (tiles of four,
+//      single units; pairs are the default): one plan each;
 //   C. for a position-weight winner, the launch-only knobs on that one plan
 //      (items per wave, XCD run length, waves per SIMD, store kind), one knob
 //      at a time in that order, each kept at its fastest value;
@@ -289,11 +290,12 @@ extern "C" fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsign
             t.split_taps = split;
             if (!st) st = consider(&t);
         }
-        if (!st && base.accumulate == FR_ACC_POS) {
-            fr_tune t = win;
-            t.tile_writes = 1;
-            st = consider(&t);
-        }
+        if (base.accumulate == FR_ACC_POS)
+            for (int tw : {1, 2}) {
+                fr_tune t = win;
+                t.tile_writes = tw;
+                if (!st) st = consider(&t);
+            }
     }
     // C: launch-only knobs of a position-weight winner, on its one plan
     if (!st && win_ms >= 0.f && win.accumulate == FR_ACC_POS) {

@@ -95,10 +95,12 @@ typedef struct {
                                     (cached), 1: streaming (nontemporal), 2: cached */
     int waves_per_simd;          /* accumulate 6: 0: automatic (4), 3 or 4 (the
                                     kernel variant compiled for that occupancy) */
-    int tile_writes;             /* accumulate 6: 0: automatic (off), 1: four
+    int tile_writes;             /* accumulate 6: 0: automatic (pairs), 1: four
                                     x-adjacent units per workgroup written out
                                     together in whole rows, 2: off (each unit
-                                    written by its wave)                        */
+                                    written by its wave), 3: pairs (a wave
+                                    grids two x-adjacent units and writes them
+                                    out together in rows of two units)     */
 } fr_tune;
 
 typedef struct {

@@ -105,7 +105,8 @@ def sparse_points(dim, N, spacing, rng, jitter=1.0):
     (3, 32, dict(tile_x=2, tile_y=2, tile_z=4)), (3, 8, dict(tile_x=4, tile_y=4, tile_z=1)),
     (3, 32, dict(tile_writes=1)), (3, 16, dict(tile_writes=1, stores=1)),
     (2, 8, {}), (2, 12, dict(tile_x=8, tile_y=4)), (2, 32, dict(tile_x=8, tile_y=8)), (2, 16, dict(tile_x=8, tile_y=2)),
-    (2, 8, dict(tile_writes=1))])
+    (2, 8, dict(tile_writes=1)), (3, 32, dict(tile_writes=2)), (3, 12, dict(tile_writes=3)),
+    (3, 1, dict(tile_writes=3)), (3, 33, dict(tile_writes=3)), (2, 12, dict(tile_writes=3))])
 def test_sparse_samples_bitwise(ctx, orc, mode, dim, coils, tune):
     skip_big_units(mode, tune, dim)
     rng = np.random.default_rng(11 + dim + coils)
@@ -133,7 +134,9 @@ def test_sparse_samples_bitwise(ctx, orc, mode, dim, coils, tune):
                                         (8, dict(tile_writes=1, tile_x=4, tile_y=4, tile_z=1)),
                                         (32, dict(stores=1, waves_per_simd=3, items_per_wave=2)),
                                         (32, dict(tile_x=8, tile_y=2, tile_z=1, items_per_wave=2)),
-                                        (16, dict(tile_x=2, tile_y=2, tile_z=4, split_taps=100))])
+                                        (16, dict(tile_x=2, tile_y=2, tile_z=4, split_taps=100)),
+                                        (32, dict(tile_writes=2)), (16, dict(tile_writes=3, split_taps=64)),
+                                        (5, dict(tile_writes=3)), (33, dict(tile_writes=3, items_per_wave=2))])
 def test_kooshball_parity_and_determinism(ctx, orc, mode, coils, tune):
     skip_big_units(mode, tune, 3)
     tn, dcf, data, N, lut = case("kooshball3d", 900, 64, coils, seed=6, image=32)
