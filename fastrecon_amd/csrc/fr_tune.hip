@@ -282,7 +282,7 @@ extern "C" fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsign
     const fr_tune def_tune = win;
     for (const fr_tune &c : fr_tune_structural_candidates(s->dim))
         if (!st) st = consider(&c);
-    // B: slice size and (position-weight) joint tile writes around the winner
+    // B: slice size and (position-weight) write-out grouping around the winner
     if (!st && win_ms >= 0.f) {
         const fr_tune base = win;
         for (int split : {256, 1024, 4096, 16384, 65536}) {

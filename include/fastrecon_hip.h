@@ -335,8 +335,9 @@ fr_status fr_tune_cache_store(const char *path, const fr_tune_key *key, const fr
 
 /* Staged autotuning search on these samples (net-new): the default plan and
  * the structural candidates; then, around the fastest, the heavy-unit slice
- * size (split_taps 256 ... 65536) and, for a position-weight plan, joint tile
- * writes; then that plan's launch-only knobs (items_per_wave 1/2/4, xcd_run
+ * size (split_taps 256 ... 65536) and, for a position-weight plan, the
+ * write-out grouping (tile_writes 1 and 2; pairs are the default); then that
+ * plan's launch-only knobs (items_per_wave 1/2/4, xcd_run
  * 64 ... 4096, waves_per_simd 3/4, stores 1/2), one at a time; finally the
  * default and the winner in turn, three rounds each, and the winner is kept
  * only if it beats the default by 1%.  *best gets the winner as an explicit
