@@ -243,6 +243,40 @@ __device__ __forceinline__ void grid_records(f4 (&acc)[NCB], uint32_t r, uint32_
     }
 }
 
+// grid_records over [r, r1) with a switch at rmid (a multiple of RB): the
+// records up to rmid belong to one unit, the rest to the next; at rmid the
+// pipeline keeps running and on_mid() moves the lane to the next unit's
+// cell (fx, fy, fz) and accumulators.
+template <int NCB, bool PAIR, bool FULL, typename TS, typename Mid>
+__device__ __forceinline__ void grid_records_mid(f4 (&acc)[NCB], uint32_t r, uint32_t rmid, uint32_t r1,
+                                                 const uint32_t *__restrict__ recs, const float4 *__restrict__ pos,
+                                                 const TS *__restrict__ data, uint32_t C, int ca, int cbb,
+                                                 const float4 *stbl, float &fx, float &fy, float &fz, float dacc,
+                                                 float cidx, float lmax, bool v0ok, bool v1ok, int j, int k, Mid on_mid)
+{
+    uint32_t id0[4], id1[4];
+    batch_st S0, S1;
+    load_ids(id0, recs, r, k);
+    load_ids(id1, recs, r + RB, k);
+    issue_batch<NCB, PAIR>(S0, id0, pos, data, C, ca, cbb);
+    while (true) {
+        load_ids(id0, recs, r + 2 * RB, k);
+        issue_batch<NCB, PAIR>(S1, id1, pos, data, C, ca, cbb);
+        __builtin_amdgcn_sched_barrier(0);
+        apply_batch<NCB, FULL>(acc, S0, stbl, fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j);
+        r += RB;
+        if (r == rmid) on_mid();
+        if (r >= r1) break;
+        load_ids(id1, recs, r + 2 * RB, k);
+        issue_batch<NCB, PAIR>(S0, id0, pos, data, C, ca, cbb);
+        __builtin_amdgcn_sched_barrier(0);
+        apply_batch<NCB, FULL>(acc, S1, stbl, fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j);
+        r += RB;
+        if (r == rmid) on_mid();
+        if (r >= r1) break;
+    }
+}
+
 // NCB component blocks (chunks of 8 NCB coils).  PAIR: C even, so lane j's
 // two coils (2j, 2j+1 of a 32-coil chunk) are one aligned 16-byte (8-byte
 // for FP16) load.  FULL: every chunk holds 8 NCB coils.
@@ -423,7 +457,6 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
         const int bxs = 1 << lbx, lb2 = lbx + 1;
         constexpr int steps = NCB == 4 ? 2 : 1;
         constexpr int ncs = NCB == 1 ? 8 : 16;        // coils per step
-        constexpr int c1 = NCB < 2 ? NCB : 2;         // component slices of the first step
         // lane (j, k): register r holds cell q = 4 k + r of a unit; unit A's
         // cells go to region x < bx, B's to x >= bx
         auto stage = [&](f4 a, int cb, int xoff) {
@@ -436,28 +469,28 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
         };
         const fr_item ita = items[ia], itb = items[ib];
         const item_box ba = unpack_box(ita.t0, ita.pad), bb = unpack_box(itb.t0, itb.pad);
-        {
-            float fx, fy, fz;
-            lane_cell(ba, j, fx, fy, fz);
+        float fx, fy, fz;
+        lane_cell(ba, j, fx, fy, fz);
+        // at the end of A: all its component slices into the transpose
+        // buffer (step h at offset h 32 RS); then B's cell and fresh
+        // accumulators
+        auto to_b = [&]() {
+            wave_sync();
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) stage(acc[cb], cb, (cb >> 1) * 32 * RS);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+            lane_cell(bb, j, fx, fy, fz);
+        };
+        if (ita.r0 < ita.r1 && itb.r0 < itb.r1 && itb.r0 == ita.r1) {
+            // A's and B's records are consecutive: one pipelined stream
+            grid_records_mid<NCB, PAIR, FULL>(acc, ita.r0, ita.r1, itb.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl,
+                                              fx, fy, fz, dacc, cidx, lmax, v0ok, v1ok, j, k, to_b);
+        } else {
             if (ita.r0 < ita.r1)
                 grid_records<NCB, PAIR, FULL>(acc, ita.r0, ita.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl, fx,
                                               fy, fz, dacc, cidx, lmax, v0ok, v1ok, j, k);
-        }
-        // A's first-step component slices into the transpose buffer now; for
-        // NCB 4 its others (coils 2j + 1) wait in registers
-        wave_sync();
-#pragma unroll
-        for (int cb = 0; cb < c1; ++cb) stage(acc[cb], cb, 0);
-        f4 hold0 = acc[0], hold1 = acc[0];
-        if constexpr (NCB == 4) {
-            hold0 = acc[2];
-            hold1 = acc[3];
-        }
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
-        {
-            float fx, fy, fz;
-            lane_cell(bb, j, fx, fy, fz);
+            to_b();
             if (itb.r0 < itb.r1)
                 grid_records<NCB, PAIR, FULL>(acc, itb.r0, itb.r1, recs, pos, data, (uint32_t)C, ca, cbb, stbl, fx,
                                               fy, fz, dacc, cidx, lmax, v0ok, v1ok, j, k);
@@ -465,22 +498,19 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
         const int cell = lane & 31;
         const int x = cell & ((2 << lbx) - 1), row = cell >> lb2;
         const size_t go = ((size_t)(ba.oz + (row >> lby)) * N + (ba.oy + (row & ((1 << lby) - 1)))) * N + (ba.ox + x);
+        wave_sync();
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) stage(acc[cb], cb, (cb >> 1) * 32 * RS + bxs);
+        wave_sync();
 #pragma unroll
         for (int h = 0; h < steps; ++h) {
-            if (h == 1) {
-                wave_sync();
-                stage(hold0, 2, 0);
-                stage(hold1, 3, 0);
-            }
-#pragma unroll
-            for (int cb = 2 * h; cb < (2 * h + 2 < NCB ? 2 * h + 2 : NCB); ++cb) stage(acc[cb], cb, bxs);
-            wave_sync();
+            const float *os = osm + h * 32 * RS;
 #pragma unroll
             for (int t = 0; t < ncs / 2; ++t) {
                 const int cs = (lane >> 5) + 2 * t;
                 const int cl = NCB == 4 ? 2 * cs + h : cs;
                 if (cl < nc) {
-                    const float2 v = make_float2(osm[(2 * cs) * RS + cell], osm[(2 * cs + 1) * RS + cell]);
+                    const float2 v = make_float2(os[(2 * cs) * RS + cell], os[(2 * cs + 1) * RS + cell]);
                     float2 *d = out + (size_t)(c0 + cl) * cells + go;
                     if (nt)
                         __builtin_nontemporal_store(__builtin_bit_cast(unsigned long long, v),
@@ -577,7 +607,7 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
     FR_CHECK(ctx, W < (int64_t(1) << 31), FR_ERR_UNSUPPORTED, "gridding launch of %lld workgroups", (long long)W);
     // per wave: the write-out transpose (16 cells x 16 NCB components, or a
     // pair's 32 cells x 32 components)
-    const int osm_floats = std::max(16 * NCB * CS, 32 * 33);
+    const int osm_floats = std::max(16 * NCB * CS, (NCB == 4 ? 64 : 32) * 33);
     const size_t lds = sizeof(float4) * p->L + sizeof(float) * PW * osm_floats;
     auto kern = k_grid_pos<NCB, PAIR, FULL, TS, OCC>;
     FR_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
