@@ -1,8 +1,15 @@
 // Context, sample containers, grid containers and copies.
 #include "fr_internal.h"
 
+#include <atomic>
 #include <cstring>
 #include <new>
+
+uint64_t fr_next_uid()
+{
+    static std::atomic<uint64_t> next{0};
+    return ++next;
+}
 
 fr_status fr_fail(fr_ctx *ctx, fr_status st, const char *fmt, ...)
 {
@@ -111,6 +118,7 @@ extern "C" fr_status fr_samples_create(fr_ctx *ctx, int dim, int64_t M, int coil
     }
     hipMemsetAsync(s->data, 0, esz * (M + 1) * coils, ctx->stream);
     hipMemsetAsync(s->dcf, 0, sizeof(float) * (M + 1), ctx->stream);
+    s->uid = fr_next_uid();
     *out = s;
     return FR_OK;
 }
@@ -174,6 +182,7 @@ extern "C" fr_status fr_samples_create_frames(fr_ctx *ctx, int frames, const int
         fr_samples_destroy(s);
         return fr_fail(ctx, FR_ERR_HIP, "frame samples: %s", hipGetErrorString(e));
     }
+    s->uid = fr_next_uid();
     *out = s;
     return FR_OK;
 }
@@ -323,6 +332,7 @@ extern "C" fr_status fr_samples_set_dcf(fr_samples *s, const float *dcf)
     FR_HIP(ctx, hipMemcpyAsync(s->dcf, dcf, sizeof(float) * s->M, hipMemcpyHostToDevice, ctx->stream));
     FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->has_dcf = true;
+    fr_dcf_changed(s);
     return FR_OK;
 }
 
@@ -334,6 +344,7 @@ extern "C" fr_status fr_samples_set_dcf_device(fr_samples *s, const float *dcf)
     if (e) return e;
     FR_HIP(ctx, hipMemcpyAsync(s->dcf, dcf, sizeof(float) * s->M, hipMemcpyDeviceToDevice, ctx->stream));
     s->has_dcf = true;
+    fr_dcf_changed(s);
     return FR_OK;
 }
 

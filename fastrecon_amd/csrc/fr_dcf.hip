@@ -255,16 +255,20 @@ extern "C" fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int
         hipLaunchKernelGGL(k_fill_f32, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, M, 1.f);
     else
         hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+    fr_dcf_changed(u);
     const int mode = density_mode(p);
     for (int it = 0; it < iters && !st; ++it) {
         st = fr_launch_grid(p, u, p->dcf_g->data, mode);
         if (!st) st = fr_degrid_raw(p, u, p->dcf_g->data, 1, p->dcf_den);
-        if (!st)
+        if (!st) {
             hipLaunchKernelGGL(k_pm_update, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, u->dcf, p->dcf_den, M);
+            fr_dcf_changed(u);
+        }
     }
     if (st) return st;
     hipMemcpyAsync(s->dcf, u->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
     s->has_dcf = true;
+    fr_dcf_changed(s);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     FR_CHECK(ctx, e == hipSuccess, FR_ERR_HIP, "pipe-menon: %s", hipGetErrorString(e));
     return FR_OK;
@@ -289,6 +293,7 @@ extern "C" fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, float *alp
     if (st) return st;
     fr_samples *u = p->dcf_u;
     hipMemcpyAsync(u->dcf, s->dcf, sizeof(float) * M, hipMemcpyDeviceToDevice, ctx->stream);
+    fr_dcf_changed(u);
     st = fr_launch_grid(p, u, p->dcf_g->data, density_mode(p));
     if (st) return st;
     // the partial sums and alpha live in the degridded-density buffer (sized
@@ -299,6 +304,7 @@ extern "C" fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, float *alp
                        (int64_t)p->dcf_g->cells, s->dcf, M, part);
     hipLaunchKernelGGL(k_identity_alpha, dim3(1), dim3(64), 0, ctx->stream, part, SUM_BLOCKS, alpha);
     hipLaunchKernelGGL(k_scale_by, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, s->dcf, M, alpha);
+    fr_dcf_changed(s);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && alpha_out)
         e = hipMemcpyAsync(alpha_out, alpha, sizeof(float), hipMemcpyDeviceToHost, ctx->stream);

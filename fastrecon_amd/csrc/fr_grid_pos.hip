@@ -651,13 +651,22 @@ fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float
 {
     FR_CHECK(p->ctx, p->recs && p->pos && p->tsel && p->geom.cells_max <= 16 && p->L <= FR_POS_MAX_L,
              FR_ERR_STATE, "position-weight gridding needs a plan built for it (accumulate 6)");
-    // the samples' DCF into the positions' .w, on every pass: the plan may
-    // serve samples handles with different DCFs, and a captured graph
-    // replays this copy with the pass
-    if (p->M > 0)
-        hipLaunchKernelGGL(k_pos_dcf, dim3((unsigned)((p->M + 255) / 256)), dim3(256), 0, p->ctx->stream, p->pos,
+    // the samples' DCF into the positions' .w when the copy there is not
+    // this handle's current DCF (another samples handle, or a changed DCF),
+    // and always under graph capture: a replay may follow DCF changes that
+    // no host call sees, so the captured pass carries its copy and the next
+    // eager pass refreshes again
+    fr_ctx *ctx = p->ctx;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FR_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (p->M > 0 && (capturing || p->pos_dcf_uid != s->uid || p->pos_dcf_version != s->dcf_version)) {
+        hipLaunchKernelGGL(k_pos_dcf, dim3((unsigned)((p->M + 255) / 256)), dim3(256), 0, ctx->stream, p->pos,
                            s->dcf, p->M);
-    FR_HIP(p->ctx, hipGetLastError());
+        FR_HIP(ctx, hipGetLastError());
+        p->pos_dcf_uid = capturing ? 0 : s->uid;
+        p->pos_dcf_version = s->dcf_version;
+    }
     return s->type == FR_SAMPLES_C64 ? launch_occ<float2>(p, s, out, partial)
                                      : launch_occ<__half2>(p, s, out, partial);
 }

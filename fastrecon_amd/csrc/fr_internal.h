@@ -37,7 +37,15 @@ struct fr_samples {
     // frame f, and traj holds a third row with the frame index (as float)
     int frames = 0;
     std::vector<int64_t> frame_off;
+    // identity of this handle (never reused) and a counter bumped on every
+    // change of dcf[]: a position-weight plan keeps a copy of the DCF with
+    // its positions and refreshes it only when these differ from its copy's
+    uint64_t uid = 0;
+    uint64_t dcf_version = 0;
 };
+
+uint64_t fr_next_uid();
+inline void fr_dcf_changed(fr_samples *s) { ++s->dcf_version; }
 
 struct fr_grids {
     fr_ctx *ctx = nullptr;
@@ -188,6 +196,7 @@ struct fr_plan {
     // pad = oz | (ux | uy << 5 | uz << 10) << 16.
     uint32_t *recs = nullptr;
     float4 *pos = nullptr;
+    uint64_t pos_dcf_uid = 0, pos_dcf_version = 0;   // whose DCF pos[].w holds (uid 0: none)
     float4 *tsel = nullptr;
     int pos_ipw = 1;                   // items per wave (light items)
     int pos_stores = 2;                // grid stores: 1 streaming, 2 cached

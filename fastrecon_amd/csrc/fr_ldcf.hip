@@ -833,6 +833,7 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
     }
     FR_HIP(ctx, hipGetLastError());
     s->has_dcf = true;
+    fr_dcf_changed(s);
     return FR_OK;
 }
 
@@ -846,6 +847,7 @@ extern "C" fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s)
                            s->M, 1, m->mu, m->sigma, s->dcf);
     FR_HIP(ctx, hipGetLastError());
     s->has_dcf = true;
+    fr_dcf_changed(s);
     return FR_OK;
 }
 

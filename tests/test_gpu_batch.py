@@ -111,6 +111,18 @@ def test_batched_recon_and_graph_replays(ctx, orc, image):
     for f in range(F):
         ref = orc.recon_f32(tns[f], dcfs[f], datas2[f], N, 4.0, lut, (image, image, 1), nthreads=4)
         assert rel_l2(rep[f], ref) <= 1e-5
+    # a new DCF written on the device after capture: the replay grids with it
+    dcfs3 = [(d * np.float32(0.5) + np.float32(0.25)).astype(np.float32) for d in dcfs]
+    dbuf = hip.DeviceBuffer(ctx, 4 * sum(len(d) for d in dcfs3))
+    dbuf.from_host(np.concatenate(dcfs3))
+    s.set_dcf_device(dbuf.ptr)
+    graph.launch()
+    ctx.sync()
+    rep = out.to_host(np.float32).reshape(F, image * image)
+    for f in range(F):
+        ref = orc.recon_f32(tns[f], dcfs3[f], datas2[f], N, 4.0, lut, (image, image, 1), nthreads=4)
+        assert rel_l2(rep[f], ref) <= 1e-5
+    dbuf.close()
     graph.close()
 
 

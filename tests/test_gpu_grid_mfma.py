@@ -262,3 +262,47 @@ def test_nan_samples_stay_in_the_units_they_touch(ctx, orc, mode):
     fin = ~bad
     assert rel_l2(got[0][fin], ref[0][fin]) <= TOL
     assert rel_l2(got[1], ref[1]) <= TOL
+
+
+def test_dcf_changes_reach_the_position_plan(ctx, orc):
+    """The position-weight plan keeps a copy of the DCF with its positions;
+    every way a DCF changes reaches the next pass: a host upload, a device
+    upload, another samples handle on the same plan, and back."""
+    C = 8
+    tn, dcf, data, N, lut = case("kooshball3d", 600, 48, C, seed=9, image=24)
+    M = tn.shape[0]
+    rng = np.random.default_rng(71)
+    s = hip.Samples(ctx, 3, M, C)
+    s.set_traj(tn)
+    s.set_dcf(dcf)
+    s.set_coils(data)
+    p = hip.Plan(ctx, s, N, 4.0, lut)
+    assert p.info()["accumulate"] == 6
+    g = hip.Grids(ctx, 3, N, C)
+
+    def check(samples, w, what):
+        hip.grid_execute(p, samples, g)
+        got = g.download_all()
+        ref = orc.grid_f32(tn, w, data, N, 4.0, lut, nthreads=8)
+        e = rel_l2(got, ref)
+        print(f"[dcf change: {what}] rel-L2 {e:.2e}", flush=True)
+        assert e <= TOL
+
+    check(s, dcf, "initial")
+    w2 = rng.uniform(0.1, 2.0, M).astype(np.float32)
+    s.set_dcf(w2)
+    check(s, w2, "host upload")
+    w3 = rng.uniform(0.1, 2.0, M).astype(np.float32)
+    buf = hip.DeviceBuffer(ctx, 4 * M)
+    buf.from_host(w3)
+    s.set_dcf_device(buf.ptr)
+    check(s, w3, "device upload")
+    s2 = hip.Samples(ctx, 3, M, C)
+    s2.set_traj(tn)
+    w4 = rng.uniform(0.1, 2.0, M).astype(np.float32)
+    s2.set_dcf(w4)
+    s2.set_coils(data)
+    check(s2, w4, "second handle")
+    check(s, w3, "first handle again")
+    for h in (buf, s2, g, p, s):
+        h.close()
