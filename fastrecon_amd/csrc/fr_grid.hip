@@ -472,7 +472,7 @@ size_t fr_grid_scratch_bytes(const fr_plan *p, int C)
 }
 
 fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode, float2 *scratch,
-                         size_t scratch_bytes)
+                         size_t scratch_bytes, const fr_samples *dcf)
 {
     fr_ctx *ctx = p->ctx;
     const fr_geom &g = p->geom;
@@ -499,14 +499,17 @@ fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force
         partial = (float2 *)p->partial;
     }
     if (p->nitems == 0) return FR_OK;
+    const int mode = force_mode ? force_mode : p->accumulate;
+    FR_CHECK(ctx, !dcf || mode == FR_ACC_POS, FR_ERR_STATE,
+             "another handle's DCF applies to position-weight gridding only");
     const bool c64 = s->type == FR_SAMPLES_C64;
     fr_status e;
-    switch (force_mode ? force_mode : p->accumulate) {
+    switch (mode) {
     case FR_ACC_MFMA:
         e = fr_launch_grid_mfma(p, s, out, partial);
         break;
     case FR_ACC_POS:
-        e = fr_launch_grid_pos(p, s, out, partial);
+        e = fr_launch_grid_pos(p, s, out, partial, 1, dcf);
         break;
     case FR_ACC_GATHER:
         e = c64 ? launch<FR_ACC_GATHER, float2>(p, s, out, partial) : launch<FR_ACC_GATHER, __half2>(p, s, out, partial);

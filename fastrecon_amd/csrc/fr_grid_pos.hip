@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     const float4 *__restrict__ tbl, int L, float dacc, float cidx, const TS *__restrict__ data,
     int C, int N, size_t cells, float2 *__restrict__ out, float2 *__restrict__ partial,
     int64_t nslots, int cells_max, int64_t nfront, int64_t front_wg, int ipw, int64_t xr, int nt, int tw,
-    int osm_floats)
+    int osm_floats, int64_t tstride, int fsub)
 {
     extern __shared__ float4 smem_pos[];
     float4 *stbl = smem_pos;                                         // L entries
@@ -332,7 +332,8 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
         // ---------------------------------------------------------- tile
         if (grp >= ntiles) return;              // the whole workgroup
         const fr_tile T = tiles[grp];
-        const int32_t myitem = T.item[wv];
+        // (a select, not T.item[wv]: a per-lane index would put T in scratch)
+        const int32_t myitem = wv == 0 ? T.item[0] : wv == 1 ? T.item[1] : wv == 2 ? T.item[2] : T.item[3];
         const int bx = 1 << lbx, by = 1 << lby;
         const int W = PW * bx, rows = by << lbz;    // tile row length, rows per coil
         float *tb = reinterpret_cast<float *>(smem_pos + L);
@@ -436,7 +437,7 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     // (8 when a chunk has 8): region cell x + 2 bx (y + by z) of the pair's
     // 2 bx x by x bz box, unit A at x < bx, unit B at x >= bx.
     if (bid >= front_wg && ntiles > 0) {
-        const int64_t pidx = grp * PW + wv;
+        const int64_t pidx = fr_sampled_tile(grp * PW + wv, tstride);   // (tstride 1: every pair)
         if (pidx >= ntiles) return;
         const fr_tile T = tiles[pidx];
         const int32_t ia = T.item[0], ib = T.item[1];
@@ -524,8 +525,11 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
     }
 
     int64_t i, iend;
+    const int64_t fw = bid * PW + wv;
     if (bid < front_wg) {
-        i = bid * PW + wv;
+        // fsub > 1 (a sample of the grid): wave fw takes sub-slice fw % fsub
+        // of front item fw / fsub
+        i = fsub > 1 ? fw / fsub : fw;
         iend = min(i + 1, nfront);
     } else {
         i = nfront + (grp * PW + wv) * ipw;
@@ -535,6 +539,18 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
 
     // current item; empty items (units without records) write zeros
     fr_item cur = items[i];
+    if (bid < front_wg && fsub > 1) {
+        // only the front units of every tstride-th rank; the sub-slice's
+        // records (a multiple of FR_POS_BATCH, possibly none) go to
+        // virtual slot fw
+        if (cur.unit % (uint32_t)tstride != 0) return;
+        const uint32_t sub = (uint32_t)(fw % fsub), len = cur.r1 - cur.r0;
+        const uint32_t sl = ((len + fsub - 1) / fsub + (FR_POS_BATCH - 1)) & ~(uint32_t)(FR_POS_BATCH - 1);
+        const uint32_t a = min(cur.r0 + sub * sl, cur.r1);
+        cur.r0 = a;
+        cur.r1 = min(a + sl, cur.r1);
+        cur.slot = (int32_t)fw;
+    }
     item_box box = unpack_box(cur.t0, cur.pad);
     while (cur.r0 == cur.r1) {
         write_out(cur, box);
@@ -586,20 +602,22 @@ __global__ void __launch_bounds__(64 * PW) __attribute__((amdgpu_waves_per_eu(OC
 }
 
 template <int NCB, bool PAIR, bool FULL, typename TS, int OCC>
-fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
+fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial, int64_t tstride, int fsub)
 {
     fr_ctx *ctx = p->ctx;
     const size_t cells = fr_plan_cells(p->dim, p->N, p->frames);
     const int chunks = (s->C + 8 * NCB - 1) / (8 * NCB);
     const int ipw = std::max(1, p->pos_ipw);
     const int64_t per_wg = (int64_t)PW * ipw;
-    const int64_t front_wg = (p->nitems_front + PW - 1) / PW;
+    // tstride > 1 (pair plans only): the sampled pairs (fr_sampled_tile), and
+    // the front items cut into fsub sub-slices each (one wave per sub-slice)
+    const int64_t front_wg = (p->nitems_front * (tstride > 1 ? fsub : 1) + PW - 1) / PW;
     const int64_t light = p->nitems - p->nitems_front;
-    // light work: tiles (one per workgroup) or runs of ipw items per wave
+    const int64_t npairs = fr_sampled_tiles(p->ntiles, tstride);
     // light work: tiles of four (one per workgroup), pairs (PW per
     // workgroup) or runs of ipw items per wave
     const bool fours = p->ntiles > 0 && p->tile_units == PW;
-    const int64_t groups = p->ntiles > 0 ? (fours ? p->ntiles : (p->ntiles + PW - 1) / PW)
+    const int64_t groups = p->ntiles > 0 ? (fours ? p->ntiles : (npairs + PW - 1) / PW)
                                          : (light + per_wg - 1) / per_wg;
     const int64_t xr = std::max<int64_t>(1, p->xcd_run / (p->ntiles > 0 ? (fours ? PW : 2 * PW) : per_wg));
     const int64_t span = 8 * xr;
@@ -617,19 +635,19 @@ fr_status launch(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
                        p->tiles, p->ntiles, lbx, lby, lbz, p->recs, p->pos, p->tsel, p->L, p->mfma_dacc, p->mfma_cidx,
                        (const TS *)s->data, s->C, (int)p->N, cells, out, partial, p->nslots,
                        p->geom.cells_max, p->nitems_front, front_wg, ipw, xr, p->pos_stores == 1 ? 1 : 0,
-                       p->tile_units, osm_floats);
+                       p->tile_units, osm_floats, tstride, tstride > 1 ? fsub : 1);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
 
 template <typename TS, int OCC>
-fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
+fr_status launch_ncb(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial, int64_t ts, int fs)
 {
-    if (s->C <= 8) return launch<1, false, false, TS, OCC>(p, s, out, partial);
-    if (s->C <= 16) return launch<2, false, false, TS, OCC>(p, s, out, partial);
-    if (s->C % 32 == 0) return launch<4, true, true, TS, OCC>(p, s, out, partial);
-    if (s->C % 2 == 0) return launch<4, true, false, TS, OCC>(p, s, out, partial);
-    return launch<4, false, false, TS, OCC>(p, s, out, partial);
+    if (s->C <= 8) return launch<1, false, false, TS, OCC>(p, s, out, partial, ts, fs);
+    if (s->C <= 16) return launch<2, false, false, TS, OCC>(p, s, out, partial, ts, fs);
+    if (s->C % 32 == 0) return launch<4, true, true, TS, OCC>(p, s, out, partial, ts, fs);
+    if (s->C % 2 == 0) return launch<4, true, false, TS, OCC>(p, s, out, partial, ts, fs);
+    return launch<4, false, false, TS, OCC>(p, s, out, partial, ts, fs);
 }
 
 // pos[i].w <- the samples' DCF (the null sample keeps 0)
@@ -640,33 +658,41 @@ __global__ void k_pos_dcf(float4 *__restrict__ pos, const float *__restrict__ dc
 }
 
 template <typename TS>
-fr_status launch_occ(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
+fr_status launch_occ(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial, int64_t ts, int fs)
 {
-    return p->pos_waves == 4 ? launch_ncb<TS, 4>(p, s, out, partial) : launch_ncb<TS, 1>(p, s, out, partial);
+    return p->pos_waves == 4 ? launch_ncb<TS, 4>(p, s, out, partial, ts, fs)
+                             : launch_ncb<TS, 1>(p, s, out, partial, ts, fs);
 }
 
 }  // namespace
 
-fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial)
+fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial, int64_t tile_stride,
+                             const fr_samples *dcf, int front_sub)
 {
     FR_CHECK(p->ctx, p->recs && p->pos && p->tsel && p->geom.cells_max <= 16 && p->L <= FR_POS_MAX_L,
              FR_ERR_STATE, "position-weight gridding needs a plan built for it (accumulate 6)");
-    // the samples' DCF into the positions' .w when the copy there is not
-    // this handle's current DCF (another samples handle, or a changed DCF),
-    // and always under graph capture: a replay may follow DCF changes that
-    // no host call sees, so the captured pass carries its copy and the next
+    FR_CHECK(p->ctx, tile_stride == 1 || (tile_stride > 1 && p->ntiles > 0 && p->tile_units == 2), FR_ERR_STATE,
+             "a tile stride needs a plan with unit pairs");
+    FR_CHECK(p->ctx, tile_stride == 1 || (front_sub >= 1 && s->C == 1), FR_ERR_INVALID,
+             "a sample of the grid is single-channel, with at least one sub-slice per front item");
+    const fr_samples *w = dcf ? dcf : s;
+    FR_CHECK(p->ctx, w->M == s->M, FR_ERR_INVALID, "the DCF's samples handle has another sample count");
+    // the DCF into the positions' .w when the copy there is not this
+    // handle's current DCF (another samples handle, or a changed DCF), and
+    // always under graph capture: a replay may follow DCF changes that no
+    // host call sees, so the captured pass carries its copy and the next
     // eager pass refreshes again
     fr_ctx *ctx = p->ctx;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     FR_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
     const bool capturing = cs != hipStreamCaptureStatusNone;
-    if (p->M > 0 && (capturing || p->pos_dcf_uid != s->uid || p->pos_dcf_version != s->dcf_version)) {
+    if (p->M > 0 && (capturing || p->pos_dcf_uid != w->uid || p->pos_dcf_version != w->dcf_version)) {
         hipLaunchKernelGGL(k_pos_dcf, dim3((unsigned)((p->M + 255) / 256)), dim3(256), 0, ctx->stream, p->pos,
-                           s->dcf, p->M);
+                           w->dcf, p->M);
         FR_HIP(ctx, hipGetLastError());
-        p->pos_dcf_uid = capturing ? 0 : s->uid;
-        p->pos_dcf_version = s->dcf_version;
+        p->pos_dcf_uid = capturing ? 0 : w->uid;
+        p->pos_dcf_version = w->dcf_version;
     }
-    return s->type == FR_SAMPLES_C64 ? launch_occ<float2>(p, s, out, partial)
-                                     : launch_occ<__half2>(p, s, out, partial);
+    return s->type == FR_SAMPLES_C64 ? launch_occ<float2>(p, s, out, partial, tile_stride, front_sub)
+                                     : launch_occ<__half2>(p, s, out, partial, tile_stride, front_sub);
 }

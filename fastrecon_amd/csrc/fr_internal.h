@@ -150,6 +150,8 @@ struct fr_plan {
     fr_samples *dcf_u = nullptr;
     fr_grids *dcf_g = nullptr;
     float2 *dcf_den = nullptr;
+    void *dcf_scr = nullptr;        // the sampled identity's sub-slice sums (grown on demand)
+    size_t dcf_scr_bytes = 0;
     int dim = 0;
     unsigned N = 0;
     int64_t M = 0;
@@ -259,10 +261,36 @@ static inline size_t fr_plan_cells(int dim, unsigned N, int frames)
 // records serve FR_ACC_MFMA when its LUT has at most FR_MFMA_MAX_L entries)
 // scratch: the heavy-unit slice partials go there (of scratch_bytes; no
 // allocation), else into the plan's own buffer, grown on demand
+// dcf: position-weight mode only (see fr_launch_grid_pos)
 fr_status fr_launch_grid(fr_plan *p, const fr_samples *s, float2 *out, int force_mode, float2 *scratch = nullptr,
-                         size_t scratch_bytes = 0);
+                         size_t scratch_bytes = 0, const fr_samples *dcf = nullptr);
 fr_status fr_launch_grid_mfma(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial);
-fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial);
+// A sample of the unit pairs with stride k: runs of FR_POS_WAVES consecutive
+// pairs (one workgroup's; neighbouring units share samples, which then come
+// from cache), one run in every k.  Sampled pair j is tile
+// fr_sampled_tile(j, k); fr_sampled_tiles(n, k) of n tiles are sampled (all
+// of them for k = 1).
+__host__ __device__ inline int64_t fr_sampled_tile(int64_t j, int64_t k)
+{
+    return (j / FR_POS_WAVES) * (FR_POS_WAVES * k) + j % FR_POS_WAVES;
+}
+__host__ __device__ inline int64_t fr_sampled_tiles(int64_t n, int64_t k)
+{
+    const int64_t run = (int64_t)FR_POS_WAVES * k, rest = n % run;
+    return (n / run) * FR_POS_WAVES + (rest < FR_POS_WAVES ? rest : FR_POS_WAVES);
+}
+// tile_stride > 1 (plans with unit pairs, single-channel samples): a sample
+// of the grid for estimates, not a gridding pass -- the sampled pairs of
+// light units (fr_sampled_tile) into `out` (other units' cells are left
+// alone), and the
+// front (heavy) units whose rank is a multiple of tile_stride, each front
+// item cut into front_sub sub-slices: wave w = item * front_sub + sub writes
+// its sums to virtual slot w of `partial` (cells q < the unit's cell count
+// at partial[w * 32 * cells_max + q]; nitems_front * front_sub slots), with
+// no slice combine.
+// dcf: the samples handle whose DCF weights the pass (same M; default s)
+fr_status fr_launch_grid_pos(fr_plan *p, const fr_samples *s, float2 *out, float2 *partial, int64_t tile_stride = 1,
+                             const fr_samples *dcf = nullptr, int front_sub = 1);
 // bytes of heavy-unit slice partials a pass of C coils needs
 size_t fr_grid_scratch_bytes(const fr_plan *p, int C);
 fr_status fr_grid_check(const fr_plan *p, const fr_samples *s, const fr_grids *out);

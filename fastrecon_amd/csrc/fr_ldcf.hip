@@ -16,9 +16,11 @@
 //   shells (a spoke's end samples) carry the largest weights and differ by
 //   one cell, which f0 resolves only to 0.002,
 //   f7 = log1p(n_0)/8, the other samples in this sample's own cell.
-// The counts come from a cell histogram and its summed-area table, so each
-// sample costs 4 box sums (8 corner reads each in 3D) whatever the density;
-// rho_max is one atomic maximum per sample in the histogram pass.
+// The counts come from a cell histogram, its summed-area table and a pass
+// over the cells that stores each cell's four box sums (r = 1, 2, 4, 0; 8
+// corner reads each in 3D, neighbouring cells sharing them) as one uint4,
+// so a sample reads one 16-byte cell record whatever the density; rho_max is
+// one atomic maximum per wave in the histogram pass.
 //
 // MLP layer 2 (64 x 64) runs on MFMA (v_mfma_f32_16x16x32_bf16): a wave
 // takes 16 samples, computes h1 = silu(x W1 + b1) for them in VGPRs already
@@ -56,6 +58,16 @@ __host__ __device__ inline uint32_t f2bf(float f)
     uint32_t u;
     memcpy(&u, &f, 4);
     return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;   // round to nearest even (finite values)
+}
+
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+
+// two floats to BF16, a in the low half: one v_cvt_pk_bf16_f32 (round to
+// nearest even, the same bits as f2bf for finite values)
+__device__ inline uint32_t pack_bf16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 
 __device__ inline float bf2f(uint32_t b) { return __uint_as_float(b << 16); }
@@ -185,32 +197,83 @@ __global__ void k_prefix_axis(uint32_t *__restrict__ a, int N, int dim, int axis
     }
 }
 
-__device__ inline uint32_t sat_at(const uint32_t *__restrict__ S, int N, int x, int y, int z)
+// inclusive prefix sum along x (contiguous lines of N): one wave per line,
+// 256 cells per step (four per lane, then a wave scan of the lane totals);
+// integer sums, so the same values as a sequential scan
+__global__ void __launch_bounds__(256) k_prefix_x(uint32_t *__restrict__ a, int N, int64_t lines)
 {
-    if (x < 0 || y < 0 || z < 0) return 0u;
-    return S[((size_t)z * N + y) * N + x];
+    const int64_t line = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (line >= lines) return;                  // whole waves
+    uint32_t *row = a + (size_t)line * N;
+    uint32_t carry = 0;
+    for (int x0 = 0; x0 < N; x0 += 256) {
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + 4 * lane + k;
+            v[k] = x < N ? row[x] : 0u;
+        }
+        v[1] += v[0];
+        v[2] += v[1];
+        v[3] += v[2];
+        uint32_t t = v[3];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(t, d);
+            if (lane >= d) t += u;
+        }
+        const uint32_t base = carry + t - v[3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + 4 * lane + k;
+            if (x < N) row[x] = base + v[k];
+        }
+        carry += __shfl(t, 63);
+    }
 }
 
-__device__ inline uint32_t box_count(const uint32_t *__restrict__ S, int dim, int N, const int *c, int r)
+// the count of the box of Chebyshev radius r around cell (x, y, z) (clipped
+// to the grid) from the summed-area table S by inclusion-exclusion of its
+// eight corners, corners outside the table (index -1) reading as zero;
+// 32-bit cell indices (cells < 2^31)
+__device__ inline uint32_t row_box(const uint32_t *__restrict__ S, int dim, int N, int x, int y, int z, int r)
 {
-    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    for (int d = 0; d < dim; ++d) {
-        lo[d] = max(c[d] - r, 0) - 1;
-        hi[d] = min(c[d] + r, N - 1);
-    }
-    if (dim == 2) {
-        return sat_at(S, N, hi[0], hi[1], 0) - sat_at(S, N, lo[0], hi[1], 0) - sat_at(S, N, hi[0], lo[1], 0) +
-               sat_at(S, N, lo[0], lo[1], 0);
-    }
-    return sat_at(S, N, hi[0], hi[1], hi[2]) - sat_at(S, N, lo[0], hi[1], hi[2]) - sat_at(S, N, hi[0], lo[1], hi[2]) -
-           sat_at(S, N, hi[0], hi[1], lo[2]) + sat_at(S, N, lo[0], lo[1], hi[2]) + sat_at(S, N, lo[0], hi[1], lo[2]) +
-           sat_at(S, N, hi[0], lo[1], lo[2]) - sat_at(S, N, lo[0], lo[1], lo[2]);
+    const int xl = max(x - r, 0) - 1, xh = min(x + r, N - 1);
+    const int yl = max(y - r, 0) - 1, yh = min(y + r, N - 1);
+    auto at = [&](int zz, int yy) -> uint32_t {      // S(xh) - S(xl) on row (yy, zz)
+        if (zz < 0 || yy < 0) return 0u;
+        const uint32_t *row = S + ((uint32_t)zz * (uint32_t)N + (uint32_t)yy) * (uint32_t)N;
+        return row[xh] - (xl >= 0 ? row[xl] : 0u);
+    };
+    if (dim == 2) return at(0, yh) - at(0, yl);
+    const int zl = max(z - r, 0) - 1, zh = min(z + r, N - 1);
+    return (at(zh, yh) - at(zh, yl)) - (at(zl, yh) - at(zl, yl));
+}
+
+// per cell: the sample counts of the boxes of Chebyshev radius 1, 2, 4 and 0
+// around it, from the summed-area table S; one workgroup per x-row y + N z.
+// Workgroups go round-robin to the eight XCDs, so workgroup b takes row
+// (b % 8) per + b / 8: each XCD walks its own run of consecutive rows, whose
+// corner rows its L2 then holds.  Unsigned sums wrap, so the grouping of the
+// eight corner terms does not change the count.
+__global__ void __launch_bounds__(256) k_box_counts(const uint32_t *__restrict__ S, int dim, int N, uint32_t lines,
+                                                    uint32_t per, uint4 *__restrict__ B)
+{
+    const uint32_t rowi = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+    if (rowi >= lines) return;
+    const int y = (int)(rowi % (uint32_t)N), z = (int)(rowi / (uint32_t)N);
+    uint4 *out = B + (size_t)rowi * N;
+    for (int x = threadIdx.x; x < N; x += blockDim.x)
+        out[x] = make_uint4(row_box(S, dim, N, x, y, z, 1), row_box(S, dim, N, x, y, z, 2),
+                            row_box(S, dim, N, x, y, z, 4), row_box(S, dim, N, x, y, z, 0));
 }
 
 // the eight features of sample s; S: summed-area table of the cell counts,
-// followed by rho_max (float bits)
+// followed by rho_max (float bits); B: the cells' box counts (k_box_counts)
 __device__ inline void sample_features(const float *__restrict__ traj, int64_t M, int dim, int N,
-                                       const uint32_t *__restrict__ S, int64_t s, float (&f)[FSTRIDE])
+                                       const uint32_t *__restrict__ S, const uint4 *__restrict__ B, int64_t s,
+                                       float (&f)[FSTRIDE])
 {
     const float cf = 0.5f * (float)(N - 1), hN = 0.5f * (float)N;
     float u[3] = {0.f, 0.f, 0.f};
@@ -227,22 +290,25 @@ __device__ inline void sample_features(const float *__restrict__ traj, int64_t M
     f[0] = log1pf(r * hN) * 0.2f;             // radius in cells, log-compressed
     f[1] = atan2f(u[1], u[0]) * inv_pi;
     f[2] = (dim == 3 && r > 0.f) ? acosf(fminf(fmaxf(u[2] / r, -1.f), 1.f)) * inv_pi : 0.5f;
-    const int rr[3] = {1, 2, 4};
-    for (int q = 0; q < 3; ++q) f[3 + q] = log1pf((float)(box_count(S, dim, N, c, rr[q]) - 1u)) * 0.125f;
+    const size_t cell = ((size_t)c[2] * N + c[1]) * N + c[0];
+    const uint4 n = B[cell];
+    f[3] = log1pf((float)(n.x - 1u)) * 0.125f;
+    f[4] = log1pf((float)(n.y - 1u)) * 0.125f;
+    f[5] = log1pf((float)(n.z - 1u)) * 0.125f;
     const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
     const float rho_max = __uint_as_float(S[ncells]);
     const float rho = __fmul_rn(r, hN);
     f[6] = fminf(__fsub_rn(rho_max, rho), 4.f) * 0.25f;
-    f[7] = log1pf((float)(box_count(S, dim, N, c, 0) - 1u)) * 0.125f;
+    f[7] = log1pf((float)(n.w - 1u)) * 0.125f;
 }
 
 __global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, const uint32_t *__restrict__ S,
-                           float *__restrict__ feat)
+                           const uint4 *__restrict__ B, float *__restrict__ feat)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     float f[FSTRIDE];
-    sample_features(traj, M, dim, N, S, s, f);
+    sample_features(traj, M, dim, N, S, B, s, f);
     float4 *o = reinterpret_cast<float4 *>(feat + s * FSTRIDE);
     o[0] = make_float4(f[0], f[1], f[2], f[3]);
     o[1] = make_float4(f[4], f[5], f[6], f[7]);
@@ -325,7 +391,7 @@ __device__ inline void mlp_forward16(const WS &W, const u32x4 (&a)[4][2], const 
     u32x4 b[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
-        uint32_t hb[8];
+        float h[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = 32 * ks + 8 * g + j;
@@ -333,10 +399,16 @@ __device__ inline void mlp_forward16(const WS &W, const u32x4 (&a)[4][2], const 
 #pragma unroll
             for (int i = 0; i < NF; ++i) z = fmaf(x[i], W.w1(i, k), z);
             F.z1[ks][j] = z;
-            hb[j] = f2bf(silu(z));
-            F.h1[ks][j] = bf2f(hb[j]);
+            h[j] = silu(z);
         }
-        b[ks] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16)};
+        uint32_t hb[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            hb[j] = pack_bf16(h[2 * j], h[2 * j + 1]);
+            F.h1[ks][2 * j] = bf2f(hb[j] & 0xffffu);
+            F.h1[ks][2 * j + 1] = bf2f(hb[j] >> 16);
+        }
+        b[ks] = u32x4{hb[0], hb[1], hb[2], hb[3]};
     }
     float part = 0.f;
 #pragma unroll
@@ -406,21 +478,20 @@ template <class WS>
 __device__ inline float mlp_infer16(const WS &W, const infer_frags &F, float x0, float x1, int lane)
 {
     const int g = lane >> 4;
-    uint32_t hb[4][4];
+    uint32_t hb[4][2];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const int u = 16 * mt + 4 * g;
         f32x4 acc = {W.b1(u), W.b1(u + 1), W.b1(u + 2), W.b1(u + 3)};
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.a1[mt][0], x0, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(F.a1[mt][1], x1, acc, 0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) hb[mt][r] = f2bf(silu(acc[r]));
+        hb[mt][0] = pack_bf16(silu(acc[0]), silu(acc[1]));
+        hb[mt][1] = pack_bf16(silu(acc[2]), silu(acc[3]));
     }
     u32x4 b[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
-        b[ks] = u32x4{hb[2 * ks][0] | (hb[2 * ks][1] << 16), hb[2 * ks][2] | (hb[2 * ks][3] << 16),
-                      hb[2 * ks + 1][0] | (hb[2 * ks + 1][1] << 16), hb[2 * ks + 1][2] | (hb[2 * ks + 1][3] << 16)};
+        b[ks] = u32x4{hb[2 * ks][0], hb[2 * ks][1], hb[2 * ks + 1][0], hb[2 * ks + 1][1]};
     float part = 0.f;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
@@ -479,8 +550,8 @@ __global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__r
 // the wave runs the network over them 16 at a time.  Same arithmetic as
 // k_features followed by k_mlp_forward.
 __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__restrict__ traj, int64_t M, int dim, int N,
-                                                  const uint32_t *__restrict__ S, int mode, float mu, float sigma,
-                                                  float *__restrict__ out)
+                                                  const uint32_t *__restrict__ S, const uint4 *__restrict__ B, int mode,
+                                                  float mu, float sigma, float *__restrict__ out)
 {
     __shared__ lds_weights LW;
     __shared__ float xs[4][64][NF + 1];
@@ -497,7 +568,7 @@ __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__rest
         const int64_t s0 = gi * 64;
         {
             float f[FSTRIDE];
-            sample_features(traj, M, dim, N, S, min(s0 + lane, M - 1), f);
+            sample_features(traj, M, dim, N, S, B, min(s0 + lane, M - 1), f);
 #pragma unroll
             for (int i = 0; i < NF; ++i) xs[wv][lane][i] = f[i];
         }
@@ -679,7 +750,7 @@ struct fr_mlp {
     int64_t partial_slots = 0;
     int64_t step = 0;
     float mu = 0.f, sigma = 1.f;
-    uint32_t *hist = nullptr;     // cell histogram / summed-area table of fr_ldcf_dcf, kept between calls
+    uint32_t *hist = nullptr;     // fr_ldcf_dcf's cell counts (counts_bytes), kept between calls
     size_t hist_cells = 0;
 };
 
@@ -689,16 +760,32 @@ mlp_dev view(const fr_mlp *m) { return mlp_dev{m->p, m->w1, m->w2, m->w3}; }
 
 extern "C" int fr_mlp_param_count(void) { return NPARAM; }
 
-// zeroed histogram -> per-cell counts -> summed-area table (in place)
-static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32_t *hist)
+// The cell-count buffer: the histogram / summed-area table (cells + 1
+// uint32, rho_max last), then from the next 16-byte boundary the cells' box
+// counts (uint4 each).
+static size_t hist_bytes(size_t cells) { return ((cells + 1) * sizeof(uint32_t) + 15) & ~(size_t)15; }
+static size_t counts_bytes(size_t cells) { return hist_bytes(cells) + cells * sizeof(uint4); }
+static uint4 *box_part(uint32_t *counts, size_t cells)
+{
+    return reinterpret_cast<uint4 *>(reinterpret_cast<char *>(counts) + hist_bytes(cells));
+}
+
+// zeroed histogram -> per-cell counts -> summed-area table (in place) -> the
+// cells' box counts
+static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32_t *counts)
 {
     // about 32 samples per thread: few enough workgroups that the central
     // box is flushed a few thousand times, not once per spoke
     const unsigned wgs = std::max(1u, std::min(nblk(s->M, 256), nblk(s->M, 256 * 32)));
-    hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist);
+    hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, counts);
     const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
-    for (int ax = 0; ax < s->dim; ++ax)
-        hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, hist, (int)N, s->dim, ax);
+    hipLaunchKernelGGL(k_prefix_x, dim3(nblk(lines, 4)), dim3(256), 0, q, counts, (int)N, lines);
+    for (int ax = 1; ax < s->dim; ++ax)
+        hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, counts, (int)N, s->dim, ax);
+    const size_t cells = fr_cells(s->dim, N);
+    const uint32_t per = (uint32_t)((lines + 7) / 8);
+    hipLaunchKernelGGL(k_box_counts, dim3(8 * per), dim3(256), 0, q, counts, s->dim, (int)N, (uint32_t)lines, per,
+                       box_part(counts, cells));
 }
 
 extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev)
@@ -707,17 +794,19 @@ extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned
     FR_CHECK(ctx, s->has_traj, FR_ERR_STATE, "trajectory not set");
     FR_CHECK(ctx, s->frames == 0, FR_ERR_UNSUPPORTED, "features of batched frames: compute them per frame");
     FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
+    FR_CHECK(ctx, fr_cells(s->dim, N) < (size_t(1) << 31), FR_ERR_UNSUPPORTED,
+             "learned-DCF features on %u^%d cells (at most 2^31 cells)", N, s->dim);
     FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_ldcf_features rows");
     const size_t cells = fr_cells(s->dim, N);
     uint32_t *hist = nullptr;
-    fr_status st = fr_alloc(ctx, (void **)&hist, (cells + 1) * sizeof(uint32_t));
+    fr_status st = fr_alloc(ctx, (void **)&hist, counts_bytes(cells));
     if (st) return st;
     hipStream_t q = ctx->stream;
     hipError_t e = hipMemsetAsync(hist, 0, (cells + 1) * sizeof(uint32_t), q);
     if (e == hipSuccess && s->M > 0) {
         launch_counts(q, s, N, hist);
         hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist,
-                           feat_dev);
+                           box_part(hist, cells), feat_dev);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(q);
@@ -813,13 +902,15 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
     FR_CHECK(ctx, s->has_traj, FR_ERR_STATE, "trajectory not set");
     FR_CHECK(ctx, s->frames == 0, FR_ERR_UNSUPPORTED, "learned DCF of batched frames: compute it per frame");
     FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
+    FR_CHECK(ctx, fr_cells(s->dim, N) < (size_t(1) << 31), FR_ERR_UNSUPPORTED,
+             "learned-DCF features on %u^%d cells (at most 2^31 cells)", N, s->dim);
     const size_t cells = fr_cells(s->dim, N);
     if (cells > m->hist_cells) {
         FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         hipFree(m->hist);
         m->hist = nullptr;
         m->hist_cells = 0;
-        fr_status st = fr_alloc(ctx, (void **)&m->hist, (cells + 1) * sizeof(uint32_t));
+        fr_status st = fr_alloc(ctx, (void **)&m->hist, counts_bytes(cells));
         if (st) return st;
         m->hist_cells = cells;
     }
@@ -829,7 +920,8 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
         const int64_t groups = (s->M + 63) / 64;
         const int64_t waves = std::min<int64_t>(groups, (int64_t)std::max(ctx->cu_count, 1) * 32);
         hipLaunchKernelGGL(k_ldcf_dcf, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0,
-                           ctx->stream, view(m), s->traj, s->M, s->dim, (int)N, m->hist, 1, m->mu, m->sigma, s->dcf);
+                           ctx->stream, view(m), s->traj, s->M, s->dim, (int)N, m->hist, box_part(m->hist, cells), 1,
+                           m->mu, m->sigma, s->dcf);
     }
     FR_HIP(ctx, hipGetLastError());
     s->has_dcf = true;

@@ -939,6 +939,7 @@ extern "C" fr_status fr_plan_destroy(fr_plan *p)
     hipFree(p->items); hipFree(p->heavy); hipFree(p->partial); hipFree(p->batches); hipFree(p->mfma_pairs);
     hipFree(p->recs); hipFree(p->pos); hipFree(p->tsel); hipFree(p->tiles);
     hipFree(p->dcf_den);
+    hipFree(p->dcf_scr);
     fr_grids_destroy(p->dcf_g);
     fr_samples_destroy(p->dcf_u);
     delete p;

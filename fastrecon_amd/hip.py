@@ -10,6 +10,7 @@ sonames as ROCm's, the copy loaded first serves both, and torch finds no
 device when ROCm's copy came first (bench.py and fastrecon_amd.ldcf.evaluate
 follow this order).
 """
+import atexit
 import ctypes as C
 import os
 import weakref
@@ -129,7 +130,7 @@ def load():
         "fr_fft_sos": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, C.c_int, _vp, _vp],
         "fr_deapod_map": [_vp, C.c_int, C.c_uint, C.c_float, _vp, C.c_int, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
         "fr_dcf_pipe_menon": [_vp, _vp, C.c_int, C.c_int],
-        "fr_dcf_identity_scale": [_vp, _vp, C.POINTER(C.c_float)],
+        "fr_dcf_identity_scale": [_vp, _vp, C.c_int, C.POINTER(C.c_float)],
         "fr_lowfilter": [_vp, _vp, C.c_int],
         "fr_normalize": [_vp, _vp],
         "fr_crop": [_vp, _vp, C.c_uint, C.c_uint, C.c_uint, _vp],
@@ -194,6 +195,21 @@ def normalize_traj(traj, N):
 
 
 # --------------------------------------------------------------- handles
+_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    """At interpreter exit, every open context closes its handles and then
+    itself, in that order, before module teardown destroys objects in an
+    arbitrary order."""
+    for ctx in list(_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device=0):
         self.L = load()
@@ -203,6 +219,7 @@ class Context:
         if st:
             raise HipError(f"fr_ctx_create({device}) failed with status {st} (is a GPU visible?)")
         self.h = h
+        _contexts.add(self)
 
     def adopt(self, child):
         """Handles are destroyed before their context (they use its stream)."""
@@ -499,11 +516,13 @@ def pipe_menon(plan, samples, iters=10, init_ones=True):
                    "fr_dcf_pipe_menon")
 
 
-def dcf_identity_scale(plan, samples):
+def dcf_identity_scale(plan, samples, tile_stride=1):
     """Scale the samples' DCF in place by alpha = sum(w) / |G w|^2 (the
-    Pipe-Menon fixed-point identity); returns alpha."""
+    Pipe-Menon fixed-point identity; |G w|^2 from every tile_stride-th unit
+    pair when > 1); returns alpha."""
     a = C.c_float(0.0)
-    plan.ctx.check(plan.ctx.L.fr_dcf_identity_scale(plan.h, samples.h, C.byref(a)), "fr_dcf_identity_scale")
+    plan.ctx.check(plan.ctx.L.fr_dcf_identity_scale(plan.h, samples.h, int(tile_stride), C.byref(a)),
+                   "fr_dcf_identity_scale")
     return float(a.value)
 
 

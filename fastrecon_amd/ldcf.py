@@ -233,13 +233,18 @@ def load(ctx, path=CHECKPOINT):
     return mlp, json.loads(str(z["meta"]))
 
 
-def learned_dcf(ctx, mlp, samples, N, scale_plan=None):
+# the identity scale's sampling stride: |G w|^2 from every 8th unit pair
+IDENTITY_STRIDE = 8
+
+
+def learned_dcf(ctx, mlp, samples, N, scale_plan=None, tile_stride=IDENTITY_STRIDE):
     """Features of the samples' (normalised) trajectory -> MLP -> samples' DCF
     (one fused pass, fr_ldcf_dcf); with `scale_plan`, then the global scale
-    of the Pipe-Menon identity (one single-channel gridding pass on that
-    plan, fr_dcf_identity_scale).  Returns the scale applied (1.0 without)."""
+    of the Pipe-Menon identity (fr_dcf_identity_scale on that plan, |G w|^2
+    from every tile_stride-th unit pair; 1: the whole grid).  Returns the
+    scale applied (1.0 without)."""
     mlp.dcf_from_traj(samples, N)
-    return hip.dcf_identity_scale(scale_plan, samples) if scale_plan is not None else 1.0
+    return hip.dcf_identity_scale(scale_plan, samples, tile_stride) if scale_plan is not None else 1.0
 
 
 def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None, coils=None):
@@ -320,8 +325,19 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
 
     t_ms = timed(learned_scaled)
     im_ms = img.to_host(np.float32)
+
+    def learned_sampled():
+        # the same with |G w|^2 estimated from every IDENTITY_STRIDE-th unit pair
+        mlp.dcf_from_traj(s, N)
+        hip.dcf_identity_scale(p, s, IDENTITY_STRIDE)
+        recon()
+
+    t_mss = timed(learned_sampled)
+    im_mss = img.to_host(np.float32)
     mlp.dcf_from_traj(s, N)
     alpha = hip.dcf_identity_scale(p, s)     # the scale the identity gives the learned DCF
+    mlp.dcf_from_traj(s, N)
+    alpha_s = hip.dcf_identity_scale(p, s, IDENTITY_STRIDE)
 
     # the iterative DCF on a second plan of the same samples with the unit
     # shape that single-channel gridding prefers (the default shape for one
@@ -347,6 +363,10 @@ def evaluate(ctx, mlp, traj, data, N, rc, ramp=None, iters=10, reps=1, log=None,
                speedup_learned_scaled_with_plan=(t_plan + t_it) / (t_plan + t_ms),
                learned_identity_scale=alpha,
                nrmse_learned_identity_scaled=nrmse(im_ms, im_it),
+               ms_learned_sampled_scale=t_mss, speedup_learned_sampled_scale=t_it / t_mss,
+               speedup_learned_sampled_scale_with_plan=(t_plan + t_it) / (t_plan + t_mss),
+               identity_stride=IDENTITY_STRIDE, learned_sampled_identity_scale=alpha_s,
+               nrmse_learned_sampled_identity_scaled=nrmse(im_mss, im_it),
                nrmse_pm1=nrmse(im_pm1, im_it), nrmse_pm1_scaled=nrmse_scaled(im_pm1, im_it),
                nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
                ms_learned_pm1=t_mr, speedup_learned_pm1=t_it / t_mr,
@@ -430,8 +450,11 @@ def evaluate_phantom(ctx, mlp, traj, N, rc, image, ramp=None, iters=10):
     im_ml = recon_image(ctx, p, s, N, rc, f, g)
     hip.dcf_identity_scale(p, s)
     im_mi = recon_image(ctx, p, s, N, rc, f, g)
+    learned_dcf(ctx, mlp, s, N, scale_plan=p)
+    im_ms = recon_image(ctx, p, s, N, rc, f, g)
     out = dict(nrmse_learned=nrmse(im_ml, im_it), nrmse_learned_scaled=nrmse_scaled(im_ml, im_it),
-               nrmse_learned_identity_scaled=nrmse(im_mi, im_it))
+               nrmse_learned_identity_scaled=nrmse(im_mi, im_it),
+               nrmse_learned_sampled_identity_scaled=nrmse(im_ms, im_it))
     if ramp is not None:
         s.set_dcf(ramp)
         im_rp = recon_image(ctx, p, s, N, rc, f, g)

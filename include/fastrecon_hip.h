@@ -279,9 +279,15 @@ fr_status fr_dcf_pipe_menon(fr_plan *p, fr_samples *s, int iters, int init_ones)
 /* Global scale of the samples' DCF (net-new) from the Pipe-Menon fixed-point
  * identity sum(w) = |G w|^2 (G^H G w = 1 at convergence): one single-channel
  * gridding of w on this plan, then w <- alpha w with alpha = sum(w)/|G w|^2,
- * exact when w is the fixed point times a constant.  alpha_out (host, may be
- * NULL) receives alpha.  Not for batched frames. */
-fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, float *alpha_out);
+ * exact when w is the fixed point times a constant.  tile_stride 1: the whole
+ * grid; k > 1 on a position-weight plan with unit pairs: |G w|^2 estimated
+ * from a sample of the grid in two strata, each sampled sum scaled by its
+ * units over its sampled units -- the light units of one run of four
+ * consecutive unit pairs in every k (a regular sub-lattice in Morton order),
+ * and the heavy (densest) units whose rank by record count is a multiple of
+ * k -- about 1/k of the gridding work (other plans take the whole grid).
+ * alpha_out (host, may be NULL) receives alpha.  Not for batched frames. */
+fr_status fr_dcf_identity_scale(fr_plan *p, fr_samples *s, int tile_stride, float *alpha_out);
 
 /* ------------------------------------------------- graphs and autotuning
  * Net-new (the reference launches per coil and partition with a device

@@ -415,3 +415,38 @@ def test_c5_2d_network_on_c1_geometry(ctx, orc):
         f"(scaled {es_ml:.4f}, identity-scaled {e_mi:.4f}), ramp {e_rp:.4f} (scaled {es_rp:.4f})")
     assert np.median(dw) <= 1e-4 and dw.max() <= 2e-2
     assert e_ml < e_rp and es_ml < es_rp
+
+
+def test_c5_shipped_network_phantom_and_psf(ctx):
+    """The shipped checkpoint on c2's geometry against the analytic 3D
+    ellipsoid phantom (single coil) and the point-spread function (all-ones
+    data): NRMSE against the 10-iteration Pipe-Menon image without a scale
+    fit, and the PSF sidelobe level (max outside a 2-pixel ball over the
+    peak), each no worse than the analytic ramp DCF's."""
+    from fastrecon_amd import ldcf
+    N, image = 256, 128
+    traj, ramp = syn.kooshball3d(51200, 256)
+    mlp, _ = ldcf.load(ctx)
+    r = ldcf.evaluate_phantom(ctx, mlp, traj, N, (image,) * 3, image, ramp=ramp)
+    tn, _ = hip.normalize_traj(traj, N)
+    M = tn.shape[0]
+    s, p = ldcf.plan_for(ctx, tn, N, 1)
+    s.set_coils(np.ones((1, M), np.complex64))
+    rc = (image,) * 3
+    side = {}
+    hip.pipe_menon(p, s, 10, True)
+    side["iterative"] = ldcf.psf_sidelobe(ldcf.recon_image(ctx, p, s, N, rc).reshape(rc))
+    mlp.dcf_from_traj(s, N)
+    side["learned"] = ldcf.psf_sidelobe(ldcf.recon_image(ctx, p, s, N, rc).reshape(rc))
+    s.set_dcf(ramp)
+    side["ramp"] = ldcf.psf_sidelobe(ldcf.recon_image(ctx, p, s, N, rc).reshape(rc))
+    for h in (p, s):
+        h.close()
+    mlp.close()
+    log(f"[c5 phantom] NRMSE vs iterative: learned {r['nrmse_learned']:.4f} (scaled {r['nrmse_learned_scaled']:.4f}, "
+        f"identity-scaled {r['nrmse_learned_identity_scaled']:.4f}), ramp {r['nrmse_ramp']:.4f} (scaled "
+        f"{r['nrmse_ramp_scaled']:.4f}); PSF sidelobe learned {side['learned']:.4f}, iterative "
+        f"{side['iterative']:.4f}, ramp {side['ramp']:.4f}")
+    assert r["nrmse_learned"] <= r["nrmse_ramp"]
+    assert r["nrmse_learned"] <= 0.06
+    assert side["learned"] <= side["ramp"]

@@ -167,6 +167,36 @@ def test_dcf_identity_scale(ctx, orc, dim):
     assert abs(a2 - 1.0) <= 1e-5
 
 
+def test_dcf_identity_scale_sampled(ctx):
+    """The identity scale with the heavy units' |G w|^2 exact and the light
+    units' from every k-th unit pair (a regular sub-lattice of the pairs):
+    within 0.5% of the whole-grid identity on a 3D kooshball (1-iteration and
+    converged Pipe-Menon DCFs, stride 4 and 8); a plan without unit pairs
+    takes the whole grid (alpha within FP32 summation-order noise)."""
+    from fastrecon_amd import ldcf
+    traj, _ = syn.kooshball3d(8000, 128)
+    N = 128
+    tn, _ = hip.normalize_traj(traj, N)
+    s, p = ldcf.plan_for(ctx, tn, N, 1)
+    s1, p1 = ldcf.plan_for(ctx, tn, N, 1, dict(tile_writes=2))
+    for iters in (1, 10):
+        hip.pipe_menon(p, s, iters, True)
+        w = s.get_dcf()
+        a = {}
+        for k in (1, 4, 8):
+            s.set_dcf(w)
+            a[k] = hip.dcf_identity_scale(p, s, k)
+        s1.set_dcf(w)
+        a_nopairs = hip.dcf_identity_scale(p1, s1, 8)
+        print(f"[identity sampled, PM-{iters}] alpha whole {a[1]:.6f}, stride 4 {a[4]:.6f}, stride 8 {a[8]:.6f}, "
+              f"no pairs {a_nopairs:.6f}", flush=True)
+        assert abs(a[4] / a[1] - 1) <= 5e-3 and abs(a[8] / a[1] - 1) <= 5e-3
+        assert a[4] != a[1] and a[8] != a[1]          # the sampled path ran
+        assert abs(a_nopairs / a[1] - 1) <= 1e-5
+    for h in (p1, s1, p, s):
+        h.close()
+
+
 @pytest.mark.parametrize("dim,N,C", [(2, 64, 4), (3, 32, 3)])
 def test_pils_ops_bitwise(ctx, orc, dim, N, C):
     """lowFilter, normalize, crop and SOS(map) are element-wise FP32 with the
