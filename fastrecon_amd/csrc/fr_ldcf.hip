@@ -15,12 +15,23 @@
 //   sample's shell (rho_max = the largest rho of all samples): the outermost
 //   shells (a spoke's end samples) carry the largest weights and differ by
 //   one cell, which f0 resolves only to 0.002,
-//   f7 = log1p(n_0)/8, the other samples in this sample's own cell.
+//   f7 = log1p(c_b / V_b)/8, the samples per cell of this sample's radius
+//   shell: b = floor(2 rho), c_b the samples (this one included) with that
+//   b, V_b the cells of the shell b/2 <= rho < b/2 + 1/2 ((4 pi/3)(hi^3 -
+//   lo^3) in 3D, pi (hi^2 - lo^2) in 2D).  A smooth density for radial-type
+//   trajectories, where the cell counts near the centre vary from cell to
+//   cell with the spoke crossings.
 // The counts come from a cell histogram, its summed-area table and a pass
-// over the cells that stores each cell's four box sums (r = 1, 2, 4, 0; 8
-// corner reads each in 3D, neighbouring cells sharing them) as one uint4,
-// so a sample reads one 16-byte cell record whatever the density; rho_max is
-// one atomic maximum per wave in the histogram pass.
+// over the cells that stores each cell's three box sums (r = 1, 2, 4; 8
+// corner reads each in 3D, neighbouring cells sharing them) in one uint4,
+// so a sample reads one 16-byte cell record whatever the density; rho_max
+// (one atomic maximum per wave) and the shell counts (per workgroup in LDS)
+// come from the histogram pass.
+//
+// The DCF head: w = exp(mu + sigma y) for the network output y, divided by
+// 1 + n_k for a count feature k = 3..5 when the network carries that base
+// (fr_mlp_set_base; exp(mu + sigma y - 8 f_k), the network then learns the
+// DCF relative to the inverse local density).
 //
 // MLP layer 2 (64 x 64) runs on MFMA (v_mfma_f32_16x16x32_bf16): a wave
 // takes 16 samples, computes h1 = silu(x W1 + b1) for them in VGPRs already
@@ -108,21 +119,36 @@ __device__ inline float sample_rho(const float *__restrict__ traj, int64_t M, in
     return __fmul_rn(sqrtf(r2), hN);
 }
 
+// radius shell of a sample: floor(2 rho), clamped to the nb shells
+__device__ inline int shell_of(float rho, int nb) { return min(max((int)floorf(2.f * rho), 0), nb - 1); }
+
 // hist[cells] (one past the histogram) receives rho_max as float bits
-// (non-negative floats order like their bits)
+// (non-negative floats order like their bits), hist[cells + 1 + b] the
+// samples of radius shell b (b < nb); with lds_shells, counted per
+// workgroup in dynamic LDS (nb words) and added once.
 __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ traj, int64_t M, int dim, int N,
-                                                   uint32_t *__restrict__ hist)
+                                                   uint32_t *__restrict__ hist, int nb, int lds_shells)
 {
     __shared__ uint32_t hot[HB * HB * HB];
+    extern __shared__ uint32_t shells[];
     const int nhot = dim == 3 ? HB * HB * HB : HB * HB;
     for (int i = threadIdx.x; i < nhot; i += blockDim.x) hot[i] = 0u;
+    if (lds_shells)
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) shells[i] = 0u;
     __syncthreads();
     const int lo = N / 2 - HR;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
+    uint32_t *gshell = hist + ncells + 1;
     float rmax = 0.f;
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < M; s += stride) {
-        rmax = fmaxf(rmax, sample_rho(traj, M, dim, N, s));
+        const float rho = sample_rho(traj, M, dim, N, s);
+        rmax = fmaxf(rmax, rho);
+        const int b = shell_of(rho, nb);
+        if (lds_shells)
+            atomicAdd(&shells[b], 1u);
+        else
+            atomicAdd(&gshell[b], 1u);
         size_t cell = 0;
         int hidx = 0;
         bool central = true;
@@ -156,6 +182,9 @@ __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ tra
         for (int d = dim - 1; d >= 0; --d) cell = cell * N + c[d];
         atomicAdd(&hist[cell], v);
     }
+    if (lds_shells)
+        for (int i = threadIdx.x; i < nb; i += blockDim.x)
+            if (shells[i]) atomicAdd(&gshell[i], shells[i]);
 }
 
 // inclusive prefix sum along one axis; axis stride `st`, lines enumerated by
@@ -251,8 +280,9 @@ __device__ inline uint32_t row_box(const uint32_t *__restrict__ S, int dim, int 
     return (at(zh, yh) - at(zh, yl)) - (at(zl, yh) - at(zl, yl));
 }
 
-// per cell: the sample counts of the boxes of Chebyshev radius 1, 2, 4 and 0
-// around it, from the summed-area table S; one workgroup per x-row y + N z.
+// per cell: the sample counts of the boxes of Chebyshev radius 1, 2 and 4
+// around it (.x, .y, .z; .w = 0), from the summed-area table S; one
+// workgroup per x-row y + N z.
 // Workgroups go round-robin to the eight XCDs, so workgroup b takes row
 // (b % 8) per + b / 8: each XCD walks its own run of consecutive rows, whose
 // corner rows its L2 then holds.  Unsigned sums wrap, so the grouping of the
@@ -266,14 +296,30 @@ __global__ void __launch_bounds__(256) k_box_counts(const uint32_t *__restrict__
     uint4 *out = B + (size_t)rowi * N;
     for (int x = threadIdx.x; x < N; x += blockDim.x)
         out[x] = make_uint4(row_box(S, dim, N, x, y, z, 1), row_box(S, dim, N, x, y, z, 2),
-                            row_box(S, dim, N, x, y, z, 4), row_box(S, dim, N, x, y, z, 0));
+                            row_box(S, dim, N, x, y, z, 4), 0u);
 }
 
-// the eight features of sample s; S: summed-area table of the cell counts,
-// followed by rho_max (float bits); B: the cells' box counts (k_box_counts)
-__device__ inline void sample_features(const float *__restrict__ traj, int64_t M, int dim, int N,
-                                       const uint32_t *__restrict__ S, const uint4 *__restrict__ B, int64_t s,
-                                       float (&f)[FSTRIDE])
+// The count tables of one trajectory: S the summed-area table of the cell
+// counts followed by rho_max (float bits) and the nb shell counts; B the
+// cells' box counts (k_box_counts).
+struct counts_view {
+    const uint32_t *S;
+    const uint4 *B;
+    int nb;
+};
+
+// cells of radius shell b (b/2 <= rho < b/2 + 1/2), FP32
+__device__ inline float shell_cells(int b, int dim)
+{
+    const float lo = 0.5f * (float)b, hi = __fadd_rn(lo, 0.5f);
+    if (dim == 3)
+        return __fmul_rn(4.18879032f, __fsub_rn(__fmul_rn(__fmul_rn(hi, hi), hi), __fmul_rn(__fmul_rn(lo, lo), lo)));
+    return __fmul_rn(3.14159274f, __fsub_rn(__fmul_rn(hi, hi), __fmul_rn(lo, lo)));
+}
+
+// the eight features of sample s
+__device__ inline void sample_features(const float *__restrict__ traj, int64_t M, int dim, int N, const counts_view &K,
+                                       int64_t s, float (&f)[FSTRIDE])
 {
     const float cf = 0.5f * (float)(N - 1), hN = 0.5f * (float)N;
     float u[3] = {0.f, 0.f, 0.f};
@@ -291,24 +337,25 @@ __device__ inline void sample_features(const float *__restrict__ traj, int64_t M
     f[1] = atan2f(u[1], u[0]) * inv_pi;
     f[2] = (dim == 3 && r > 0.f) ? acosf(fminf(fmaxf(u[2] / r, -1.f), 1.f)) * inv_pi : 0.5f;
     const size_t cell = ((size_t)c[2] * N + c[1]) * N + c[0];
-    const uint4 n = B[cell];
+    const uint4 n = K.B[cell];
     f[3] = log1pf((float)(n.x - 1u)) * 0.125f;
     f[4] = log1pf((float)(n.y - 1u)) * 0.125f;
     f[5] = log1pf((float)(n.z - 1u)) * 0.125f;
     const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
-    const float rho_max = __uint_as_float(S[ncells]);
+    const float rho_max = __uint_as_float(K.S[ncells]);
     const float rho = __fmul_rn(r, hN);
     f[6] = fminf(__fsub_rn(rho_max, rho), 4.f) * 0.25f;
-    f[7] = log1pf((float)(n.w - 1u)) * 0.125f;
+    const int b = shell_of(rho, K.nb);
+    f[7] = log1pf(__fdiv_rn((float)K.S[ncells + 1 + b], shell_cells(b, dim))) * 0.125f;
 }
 
-__global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, const uint32_t *__restrict__ S,
-                           const uint4 *__restrict__ B, float *__restrict__ feat)
+__global__ void k_features(const float *__restrict__ traj, int64_t M, int dim, int N, counts_view K,
+                           float *__restrict__ feat)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     float f[FSTRIDE];
-    sample_features(traj, M, dim, N, S, B, s, f);
+    sample_features(traj, M, dim, N, K, s, f);
     float4 *o = reinterpret_cast<float4 *>(feat + s * FSTRIDE);
     o[0] = make_float4(f[0], f[1], f[2], f[3]);
     o[1] = make_float4(f[4], f[5], f[6], f[7]);
@@ -518,14 +565,17 @@ __device__ inline void load_x(const float *__restrict__ feat, int64_t row, float
     x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w; x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
 }
 
-__device__ inline float mlp_out(float y, int mode, float mu, float sigma)
+// mode 0: y; mode 1: the DCF exp(mu + sigma y - lb), lb = 8 x_k for a
+// network with base feature k (log(1 + n_k)), else 0
+__device__ inline float mlp_out(float y, int mode, float mu, float sigma, float lb)
 {
-    return mode == 0 ? y : __expf(__fadd_rn(mu, __fmul_rn(sigma, y)));
+    return mode == 0 ? y : __expf(__fsub_rn(__fadd_rn(mu, __fmul_rn(sigma, y)), lb));
 }
 
-// y for every sample; mode 0 writes y, mode 1 writes exp(mu + sigma y)
+// y for every sample; mode 0 writes y, mode 1 the DCF (mlp_out; base: the
+// base feature's index, -1 for none)
 __global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__restrict__ feat, int64_t M, int mode,
-                                                     float mu, float sigma, float *__restrict__ out)
+                                                     float mu, float sigma, int base, float *__restrict__ out)
 {
     __shared__ lds_weights LW;
     stage_weights(W, LW);
@@ -541,7 +591,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__r
         const int64_t s = gi * 16 + (lane & 15);
         const float *row = feat + (s < M ? s : M - 1) * FSTRIDE;
         const float y = mlp_infer16(ws, F, row[g], row[4 + g], lane);
-        if (s < M && lane < 16) out[s] = mlp_out(y, mode, mu, sigma);
+        if (s < M && lane < 16) out[s] = mlp_out(y, mode, mu, sigma, base >= 0 ? 8.f * row[base] : 0.f);
     }
 }
 
@@ -550,8 +600,8 @@ __global__ void __launch_bounds__(256) k_mlp_forward(mlp_dev W, const float *__r
 // the wave runs the network over them 16 at a time.  Same arithmetic as
 // k_features followed by k_mlp_forward.
 __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__restrict__ traj, int64_t M, int dim, int N,
-                                                  const uint32_t *__restrict__ S, const uint4 *__restrict__ B, int mode,
-                                                  float mu, float sigma, float *__restrict__ out)
+                                                  counts_view K, int mode, float mu, float sigma, int base,
+                                                  float *__restrict__ out)
 {
     __shared__ lds_weights LW;
     __shared__ float xs[4][64][NF + 1];
@@ -568,7 +618,7 @@ __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__rest
         const int64_t s0 = gi * 64;
         {
             float f[FSTRIDE];
-            sample_features(traj, M, dim, N, S, B, min(s0 + lane, M - 1), f);
+            sample_features(traj, M, dim, N, K, min(s0 + lane, M - 1), f);
 #pragma unroll
             for (int i = 0; i < NF; ++i) xs[wv][lane][i] = f[i];
         }
@@ -579,7 +629,7 @@ __global__ void __launch_bounds__(256) k_ldcf_dcf(mlp_dev W, const float *__rest
             const int r = 16 * q + (lane & 15);
             const float y = mlp_infer16(ws, F, xs[wv][r][g], xs[wv][r][4 + g], lane);
             const int64_t s = s0 + r;
-            if (s < M && lane < 16) out[s] = mlp_out(y, mode, mu, sigma);
+            if (s < M && lane < 16) out[s] = mlp_out(y, mode, mu, sigma, base >= 0 ? 8.f * xs[wv][r][base] : 0.f);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -750,8 +800,9 @@ struct fr_mlp {
     int64_t partial_slots = 0;
     int64_t step = 0;
     float mu = 0.f, sigma = 1.f;
-    uint32_t *hist = nullptr;     // fr_ldcf_dcf's cell counts (counts_bytes), kept between calls
-    size_t hist_cells = 0;
+    int base = -1;                // base feature of the DCF head (fr_mlp_set_base), -1: none
+    uint32_t *hist = nullptr;     // fr_ldcf_dcf's count tables (counts_layout), kept between calls
+    size_t hist_bytes = 0;
 };
 
 namespace {
@@ -760,32 +811,51 @@ mlp_dev view(const fr_mlp *m) { return mlp_dev{m->p, m->w1, m->w2, m->w3}; }
 
 extern "C" int fr_mlp_param_count(void) { return NPARAM; }
 
-// The cell-count buffer: the histogram / summed-area table (cells + 1
-// uint32, rho_max last), then from the next 16-byte boundary the cells' box
+// The count tables of a trajectory in one buffer: the histogram /
+// summed-area table (cells uint32), rho_max, the shell counts (nb uint32;
+// radii up to N/2 sqrt(dim) in half cells, plus margin), zeroed before each
+// use (zero_bytes); then from the next 16-byte boundary the cells' box
 // counts (uint4 each).
-static size_t hist_bytes(size_t cells) { return ((cells + 1) * sizeof(uint32_t) + 15) & ~(size_t)15; }
-static size_t counts_bytes(size_t cells) { return hist_bytes(cells) + cells * sizeof(uint4); }
-static uint4 *box_part(uint32_t *counts, size_t cells)
+struct counts_layout {
+    size_t cells, zero_bytes, box_off, bytes;
+    int nb;
+};
+static counts_layout layout_of(int dim, unsigned N)
 {
-    return reinterpret_cast<uint4 *>(reinterpret_cast<char *>(counts) + hist_bytes(cells));
+    counts_layout L;
+    L.cells = fr_cells(dim, N);
+    L.nb = (int)std::ceil((double)N * std::sqrt((double)dim)) + 2;
+    L.zero_bytes = (L.cells + 1 + (size_t)L.nb) * sizeof(uint32_t);
+    L.box_off = (L.zero_bytes + 15) & ~(size_t)15;
+    L.bytes = L.box_off + L.cells * sizeof(uint4);
+    return L;
+}
+static counts_view view_of(const uint32_t *counts, const counts_layout &L)
+{
+    return counts_view{counts, reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(counts) + L.box_off),
+                       L.nb};
 }
 
-// zeroed histogram -> per-cell counts -> summed-area table (in place) -> the
-// cells' box counts
-static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32_t *counts)
+// shells counted in LDS up to this many (larger 2D grids count in memory)
+constexpr int LDS_SHELLS = 8192;
+
+// zeroed tables -> per-cell and per-shell counts -> summed-area table (in
+// place) -> the cells' box counts
+static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32_t *counts, const counts_layout &L)
 {
     // about 32 samples per thread: few enough workgroups that the central
     // box is flushed a few thousand times, not once per spoke
     const unsigned wgs = std::max(1u, std::min(nblk(s->M, 256), nblk(s->M, 256 * 32)));
-    hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, counts);
+    const int lds = L.nb <= LDS_SHELLS;
+    hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), lds ? sizeof(uint32_t) * L.nb : 0, q, s->traj, s->M, s->dim,
+                       (int)N, counts, L.nb, lds);
     const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
     hipLaunchKernelGGL(k_prefix_x, dim3(nblk(lines, 4)), dim3(256), 0, q, counts, (int)N, lines);
     for (int ax = 1; ax < s->dim; ++ax)
         hipLaunchKernelGGL(k_prefix_axis, dim3(nblk(lines, 256)), dim3(256), 0, q, counts, (int)N, s->dim, ax);
-    const size_t cells = fr_cells(s->dim, N);
     const uint32_t per = (uint32_t)((lines + 7) / 8);
     hipLaunchKernelGGL(k_box_counts, dim3(8 * per), dim3(256), 0, q, counts, s->dim, (int)N, (uint32_t)lines, per,
-                       box_part(counts, cells));
+                       const_cast<uint4 *>(view_of(counts, L).B));
 }
 
 extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev)
@@ -797,16 +867,16 @@ extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned
     FR_CHECK(ctx, fr_cells(s->dim, N) < (size_t(1) << 31), FR_ERR_UNSUPPORTED,
              "learned-DCF features on %u^%d cells (at most 2^31 cells)", N, s->dim);
     FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_ldcf_features rows");
-    const size_t cells = fr_cells(s->dim, N);
+    const counts_layout L = layout_of(s->dim, N);
     uint32_t *hist = nullptr;
-    fr_status st = fr_alloc(ctx, (void **)&hist, counts_bytes(cells));
+    fr_status st = fr_alloc(ctx, (void **)&hist, L.bytes);
     if (st) return st;
     hipStream_t q = ctx->stream;
-    hipError_t e = hipMemsetAsync(hist, 0, (cells + 1) * sizeof(uint32_t), q);
+    hipError_t e = hipMemsetAsync(hist, 0, L.zero_bytes, q);
     if (e == hipSuccess && s->M > 0) {
-        launch_counts(q, s, N, hist);
-        hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N, hist,
-                           box_part(hist, cells), feat_dev);
+        launch_counts(q, s, N, hist, L);
+        hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N,
+                           view_of(hist, L), feat_dev);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(q);
@@ -864,6 +934,17 @@ extern "C" fr_status fr_mlp_set_params(fr_mlp *m, const float *params, float mu,
     return FR_OK;
 }
 
+extern "C" fr_status fr_mlp_set_base(fr_mlp *m, int feature)
+{
+    if (!m) return FR_ERR_INVALID;
+    FR_CHECK(m->ctx, feature == -1 || (feature >= 3 && feature <= 5), FR_ERR_INVALID,
+             "the DCF head's base is a box-count feature (3, 4 or 5) or -1, not %d", feature);
+    m->base = feature;
+    return FR_OK;
+}
+
+extern "C" int fr_mlp_get_base(const fr_mlp *m) { return m ? m->base : -1; }
+
 extern "C" fr_status fr_mlp_get_params(fr_mlp *m, float *params, float *mu, float *sigma)
 {
     if (!m || !params) return FR_ERR_INVALID;
@@ -890,7 +971,7 @@ extern "C" fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M,
     FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)M, "fr_mlp_forward feature rows");
     FR_EXTENT(ctx, y_dev, sizeof(float) * (size_t)M, "fr_mlp_forward output");
     hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, M)), dim3(256), 0, ctx->stream, view(m), feat_dev, M, 0,
-                       0.f, 1.f, y_dev);
+                       0.f, 1.f, -1, y_dev);
     FR_HIP(ctx, hipGetLastError());
     return FR_OK;
 }
@@ -904,24 +985,24 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
     FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
     FR_CHECK(ctx, fr_cells(s->dim, N) < (size_t(1) << 31), FR_ERR_UNSUPPORTED,
              "learned-DCF features on %u^%d cells (at most 2^31 cells)", N, s->dim);
-    const size_t cells = fr_cells(s->dim, N);
-    if (cells > m->hist_cells) {
+    const counts_layout L = layout_of(s->dim, N);
+    if (L.bytes > m->hist_bytes) {
         FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         hipFree(m->hist);
         m->hist = nullptr;
-        m->hist_cells = 0;
-        fr_status st = fr_alloc(ctx, (void **)&m->hist, counts_bytes(cells));
+        m->hist_bytes = 0;
+        fr_status st = fr_alloc(ctx, (void **)&m->hist, L.bytes);
         if (st) return st;
-        m->hist_cells = cells;
+        m->hist_bytes = L.bytes;
     }
     if (s->M > 0) {
-        FR_HIP(ctx, hipMemsetAsync(m->hist, 0, (cells + 1) * sizeof(uint32_t), ctx->stream));
-        launch_counts(ctx->stream, s, N, m->hist);
+        FR_HIP(ctx, hipMemsetAsync(m->hist, 0, L.zero_bytes, ctx->stream));
+        launch_counts(ctx->stream, s, N, m->hist, L);
         const int64_t groups = (s->M + 63) / 64;
         const int64_t waves = std::min<int64_t>(groups, (int64_t)std::max(ctx->cu_count, 1) * 32);
         hipLaunchKernelGGL(k_ldcf_dcf, dim3((unsigned)std::max<int64_t>(1, (waves + 3) / 4)), dim3(256), 0,
-                           ctx->stream, view(m), s->traj, s->M, s->dim, (int)N, m->hist, box_part(m->hist, cells), 1,
-                           m->mu, m->sigma, s->dcf);
+                           ctx->stream, view(m), s->traj, s->M, s->dim, (int)N, view_of(m->hist, L), 1, m->mu,
+                           m->sigma, m->base, s->dcf);
     }
     FR_HIP(ctx, hipGetLastError());
     s->has_dcf = true;
@@ -936,7 +1017,7 @@ extern "C" fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s)
     FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_mlp_dcf feature rows");
     if (s->M > 0)
         hipLaunchKernelGGL(k_mlp_forward, dim3(fwd_blocks(ctx, s->M)), dim3(256), 0, ctx->stream, view(m), feat_dev,
-                           s->M, 1, m->mu, m->sigma, s->dcf);
+                           s->M, 1, m->mu, m->sigma, m->base, s->dcf);
     FR_HIP(ctx, hipGetLastError());
     s->has_dcf = true;
     fr_dcf_changed(s);

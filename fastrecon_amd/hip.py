@@ -41,7 +41,8 @@ EXPORTS = [
     "fr_graph_create", "fr_graph_launch", "fr_graph_destroy", "fr_autotune",
     "fr_tune_key_of", "fr_tune_cache_lookup", "fr_tune_cache_store", "fr_autotune_search",
     "fr_ldcf_features", "fr_mlp_param_count", "fr_mlp_create", "fr_mlp_destroy", "fr_mlp_set_params",
-    "fr_mlp_get_params", "fr_mlp_forward", "fr_mlp_dcf", "fr_ldcf_dcf", "fr_mlp_grad", "fr_mlp_train_step",
+    "fr_mlp_get_params", "fr_mlp_set_base", "fr_mlp_get_base", "fr_mlp_forward", "fr_mlp_dcf", "fr_ldcf_dcf",
+    "fr_mlp_grad", "fr_mlp_train_step",
     "fr_malloc", "fr_free", "fr_memcpy_d2h", "fr_memcpy_h2d",
 ]
 
@@ -141,6 +142,8 @@ def load():
         "fr_mlp_create": [_vp, _pp],
         "fr_mlp_set_params": [_vp, _vp, C.c_float, C.c_float],
         "fr_mlp_get_params": [_vp, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "fr_mlp_set_base": [_vp, C.c_int],
+        "fr_mlp_get_base": [_vp],
         "fr_mlp_forward": [_vp, _vp, C.c_int64, _vp],
         "fr_mlp_dcf": [_vp, _vp, _vp],
         "fr_ldcf_dcf": [_vp, _vp, C.c_uint],
@@ -587,6 +590,16 @@ class MLP:
         mu, sg = C.c_float(), C.c_float()
         self.ctx.check(self.ctx.L.fr_mlp_get_params(self.h, _ptr(p), C.byref(mu), C.byref(sg)), "fr_mlp_get_params")
         return p, mu.value, sg.value
+
+    @property
+    def base(self):
+        """Base feature of the DCF head (3..5: the DCF is predicted relative
+        to 1 / (1 + n_k)), or -1."""
+        return int(self.ctx.L.fr_mlp_get_base(self.h))
+
+    @base.setter
+    def base(self, feature):
+        self.ctx.check(self.ctx.L.fr_mlp_set_base(self.h, int(feature)), "fr_mlp_set_base")
 
     def forward(self, feat, M):
         y = DeviceBuffer(self.ctx, 4 * max(M, 1))

@@ -185,10 +185,15 @@ def _init_params(seed):
     return np.concatenate([W1.ravel(), np.zeros(h), W2.ravel(), np.zeros(h), w3, [0.0]]).astype(np.float32)
 
 
-def train(ctx, X, W, steps=3000, batch=65536, lr=3e-3, seed=11, log=print, log_every=500):
-    """Adam on the MSE of the normalised log DCF, all on the device.
-    Returns (mlp, mu, sigma, losses)."""
+def train(ctx, X, W, steps=3000, batch=65536, lr=3e-3, seed=11, log=print, log_every=500, base=-1):
+    """Adam on the MSE of the normalised log DCF, all on the device; with a
+    base feature k (3..5) the target is log(w (1 + n_k)) = log w + 8 f_k,
+    the DCF relative to the inverse local density, and the network's head
+    divides by 1 + n_k again (fr_mlp_set_base).  Returns (mlp, mu, sigma,
+    losses)."""
     logw = np.log(np.maximum(W, 1e-30))
+    if base >= 0:
+        logw = logw + 8.0 * X[:, base].astype(np.float64)
     mu, sigma = float(logw.mean()), float(logw.std())
     t = ((logw - mu) / sigma).astype(np.float32)
     n = len(t)
@@ -199,6 +204,7 @@ def train(ctx, X, W, steps=3000, batch=65536, lr=3e-3, seed=11, log=print, log_e
     idx = hip.DeviceBuffer(ctx, 8 * n)
     mlp = hip.MLP(ctx)
     mlp.set_params(_init_params(seed), mu, sigma)
+    mlp.base = base
     rng = np.random.default_rng(seed)
     losses = []
     per_epoch = max(1, n // batch)
@@ -223,13 +229,15 @@ def train(ctx, X, W, steps=3000, batch=65536, lr=3e-3, seed=11, log=print, log_e
 
 def save(mlp, path=CHECKPOINT, meta=None):
     p, mu, sigma = mlp.get_params()
-    np.savez(path, params=p, mu=np.float32(mu), sigma=np.float32(sigma), meta=json.dumps(meta or {}))
+    np.savez(path, params=p, mu=np.float32(mu), sigma=np.float32(sigma), base=np.int32(mlp.base),
+             meta=json.dumps(meta or {}))
 
 
 def load(ctx, path=CHECKPOINT):
     z = np.load(path, allow_pickle=False)
     mlp = hip.MLP(ctx)
     mlp.set_params(z["params"], float(z["mu"]), float(z["sigma"]))
+    mlp.base = int(z["base"]) if "base" in z.files else -1
     return mlp, json.loads(str(z["meta"]))
 
 

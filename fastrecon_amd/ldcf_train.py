@@ -1,6 +1,7 @@
 """Train the learned-DCF network on the device and write the checkpoint.
 
-    python -m fastrecon_amd.ldcf_train [--dim 3] [--out PATH] [--count 200] [--per-traj 50000] [--steps 48000]
+    python -m fastrecon_amd.ldcf_train [--dim 3] [--out PATH] [--count 200] [--per-traj 50000] [--steps 96000]
+                                       [--base 4]
 
 Training set (--dim 3): `count` seeded variable-density 3D radial and cones
 trajectories for a 128^3 image (256^3 grid); (--dim 2) 2D radial and
@@ -8,7 +9,9 @@ variable-density spiral trajectories for a 256^2 image (512^2 grid); with
 10-iteration Pipe-Menon
 DCFs from this package's gridding/degridding kernels as targets
 (fastrecon_amd.ldcf.build_dataset; rows drawn by ldcf.training_rows).  The network trains with the package's
-own BF16 forward/backward kernels and Adam (cosine learning-rate decay)."""
+own BF16 forward/backward kernels and Adam (cosine learning-rate decay); with
+--base k the network learns the DCF relative to 1 / (1 + n_k) (feature k, a
+box count), which its head divides out again."""
 import argparse
 import json
 import os
@@ -29,7 +32,9 @@ def main():
     ap.add_argument("--count", type=int, default=200)
     ap.add_argument("--per-traj", type=int, default=50000)
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--steps", type=int, default=48000)
+    ap.add_argument("--steps", type=int, default=96000)
+    ap.add_argument("--base", type=int, default=4, choices=(-1, 3, 4, 5),
+                    help="box-count feature the DCF head divides by (1 + n_k); -1: none")
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=5e-3)
     ap.add_argument("--seed", type=int, default=7)
@@ -49,14 +54,15 @@ def main():
     t_data = time.time() - t0
     log(f"[ldcf] dataset {X.shape[0]} rows in {t_data:.1f} s")
     t0 = time.time()
-    net, mu, sigma, losses = ldcf.train(ctx, X, W, a.steps, a.batch, a.lr, log=log, log_every=250)
+    net, mu, sigma, losses = ldcf.train(ctx, X, W, a.steps, a.batch, a.lr, log=log, log_every=250, base=a.base)
     ctx.sync()
     t_train = time.time() - t0
     meta = dict(count=a.count, per_traj=a.per_traj, scale=a.scale, steps=a.steps, batch=a.batch, lr=a.lr,
                 seed=a.seed, rows=int(X.shape[0]), dataset_s=round(t_data, 1), train_s=round(t_train, 1),
                 losses=[(int(s), float(l)) for s, l in losses], dim=a.dim,
                 grid=ldcf.N_GRID if a.dim == 3 else ldcf.N_GRID_2D, image=ldcf.IMAGE if a.dim == 3 else ldcf.IMAGE_2D,
-                targets="10-iteration Pipe-Menon DCF", features="|u|, azimuth, polar, log1p(n_1,2,4)/8, outer-shell distance, log1p(n_0)/8",
+                targets="10-iteration Pipe-Menon DCF", base=a.base,
+                features="|u|, azimuth, polar, log1p(n_1,2,4)/8, outer-shell distance, log1p(shell density)/8",
                 row_strata="1/2 radius-stratified, 1/4 log-radius-stratified, 1/4 drawn ~ w^2")
     ldcf.save(net, out, meta)
     log(f"[ldcf] trained in {t_train:.1f} s, final loss {losses[-1][1]:.5f}; wrote {out}")

@@ -358,26 +358,35 @@ fr_status fr_autotune_search(fr_ctx *ctx, const fr_samples *s, unsigned N, float
 /* --------------------------------------------- learned density compensation
  * Net-new (the reference has no DCF estimation).  Features per sample, 8
  * floats per row: log1p(rho)/5, azimuth/pi, polar/pi, log1p(n_1)/8,
- * log1p(n_2)/8, log1p(n_4)/8, min(rho_max - rho, 4)/4, log1p(n_0)/8, with
- * u = (t - (N-1)/2)/(N/2) for the samples' normalised trajectory t,
- * rho = |u| N/2 (rho_max its largest value over the samples) and n_r the
+ * log1p(n_2)/8, log1p(n_4)/8, min(rho_max - rho, 4)/4, log1p(c_b/V_b)/8,
+ * with u = (t - (N-1)/2)/(N/2) for the samples' normalised trajectory t,
+ * rho = |u| N/2 (rho_max its largest value over the samples), n_r the
  * number of other samples whose nearest cell lies within Chebyshev distance
- * r cells.  feat_dev: M*8 floats. */
+ * r cells, and c_b / V_b the samples per cell of the sample's radius shell
+ * b = floor(2 rho) (c_b samples with that b, V_b the cells of the shell
+ * b/2 <= rho < (b+1)/2: (4 pi/3)(hi^3 - lo^3) in 3D, pi (hi^2 - lo^2) in
+ * 2D).  feat_dev: M*8 floats. */
 fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned N, float *feat_dev);
 
-/* MLP 6 -> 64 -> 64 -> 1 (SiLU), BF16 weight matrices, FP32 biases and
+/* MLP 8 -> 64 -> 64 -> 1 (SiLU), BF16 weight matrices, FP32 biases and
  * accumulation.  Parameters are a flat FP32 vector of fr_mlp_param_count()
- * values: W1[6][64], b1[64], W2[64][64] ([in][out]), b2[64], w3[64], b3.  The
- * predicted DCF is exp(mu + sigma * y). */
+ * values: W1[8][64], b1[64], W2[64][64] ([in][out]), b2[64], w3[64], b3.  The
+ * predicted DCF is exp(mu + sigma * y), divided by 1 + n_k when the network
+ * has a base feature k (fr_mlp_set_base): exp(mu + sigma * y - 8 f_k). */
 typedef struct fr_mlp fr_mlp;
 int fr_mlp_param_count(void);
 fr_status fr_mlp_create(fr_ctx *ctx, fr_mlp **out);
 void fr_mlp_destroy(fr_mlp *m);
 fr_status fr_mlp_set_params(fr_mlp *m, const float *params, float mu, float sigma);
 fr_status fr_mlp_get_params(fr_mlp *m, float *params, float *mu, float *sigma);
+/* Base feature of the DCF head: a box-count feature k = 3, 4 or 5 (the
+ * network then predicts the DCF relative to 1 / (1 + n_k), the inverse local
+ * density), or -1 for none (the default). */
+fr_status fr_mlp_set_base(fr_mlp *m, int feature);
+int fr_mlp_get_base(const fr_mlp *m);
 /* y_dev[M] = network output for feature rows feat_dev[M*8]. */
 fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M, float *y_dev);
-/* The samples' DCF <- exp(mu + sigma * y) (feature rows of the same samples). */
+/* The samples' DCF <- the DCF head (feature rows of the same samples). */
 fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s);
 /* fr_ldcf_features + fr_mlp_dcf in one pass from the samples' trajectory on
  * an N-point grid, without feature rows in memory (bitwise the same DCF).

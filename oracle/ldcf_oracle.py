@@ -15,13 +15,18 @@ ReconData::normalizeTraj, ReconData.cpp:211-225), with cf = (N-1)/2, h = N/2:
   f3..5 = log1p(n_r) / 8
   rho   = |u| N/2 (FP32: sqrt, then times N/2), rho_max = max over samples
   f6    = min(rho_max - rho, 4) / 4           distance to the outermost shell
-  f7    = log1p(n_0) / 8                      other samples in the same cell
-The counts are pinned by brute force O(M^2) here.
+  b     = min(floor(2 rho), nb - 1), nb = ceil(N sqrt(dim)) + 2
+  c_b   = #{samples j : b_j = b}              (this sample included)
+  V_b   = (4 pi / 3)(hi^3 - lo^3) in 3D, pi (hi^2 - lo^2) in 2D, lo = b/2,
+          hi = lo + 1/2                       cells of the radius shell
+  f7    = log1p(c_b / V_b) / 8                samples per cell of the shell
+The box counts are pinned by brute force O(M^2) here.
 
-MLP 6 -> 64 -> 64 -> 1 with SiLU, BF16 weight matrices, FP32 biases:
+MLP 8 -> 64 -> 64 -> 1 with SiLU, BF16 weight matrices, FP32 biases:
   z1 = x W1 + b1, h1 = bf16(silu(z1)), z2 = h1 W2 + b2, h2 = silu(z2),
   y  = h2 w3 + b3,
-with W1, W2, w3 the BF16-rounded values.  The backward restates the chain rule
+with W1, W2, w3 the BF16-rounded values.  DCF head (dcf_head): w = exp(mu +
+sigma y), divided by 1 + n_k = exp(8 f_k) for a network with base feature k.  The backward restates the chain rule
 (straight through the BF16 rounding of h1) and is pinned by central finite
 differences of the same function in FP64 without rounding.
 """
@@ -66,9 +71,35 @@ def features(tn, N):
         f[:, 3 + q] = np.log1p(n) / 8.0
     rho = (r * h).astype(np.float32)
     f[:, 6] = np.minimum((rho.max() - rho).astype(np.float32), np.float32(4.0)).astype(np.float64) * 0.25
-    n0 = np.array([np.count_nonzero(np.all(cell == cell[i], axis=1)) - 1 for i in range(M)])
-    f[:, 7] = np.log1p(n0) / 8.0
+    f[:, 7] = shell_density(rho, N, d)
     return f
+
+
+def shell_bins(N, dim):
+    return int(np.ceil(N * np.sqrt(dim))) + 2
+
+
+def shell_cells(b, dim):
+    """Cells of the radius shell b/2 <= rho < b/2 + 1/2 (FP64)."""
+    lo = np.asarray(b, np.float64) / 2.0
+    hi = lo + 0.5
+    return 4.0 * np.pi / 3.0 * (hi ** 3 - lo ** 3) if dim == 3 else np.pi * (hi ** 2 - lo ** 2)
+
+
+def shell_density(rho, N, dim):
+    """f7: log1p(samples per cell of each sample's radius shell) / 8."""
+    nb = shell_bins(N, dim)
+    b = np.clip(np.floor(2.0 * np.asarray(rho, np.float64)), 0, nb - 1).astype(np.int64)
+    c = np.bincount(b, minlength=nb)
+    return np.log1p(c[b] / shell_cells(b, dim)) / 8.0
+
+
+def dcf_head(y, x, mu, sigma, base=-1):
+    """The DCF from network outputs y of feature rows x."""
+    e = mu + sigma * np.asarray(y, np.float64)
+    if base >= 0:
+        e = e - 8.0 * np.asarray(x, np.float64)[:, base]
+    return np.exp(e)
 
 
 def neighbour_counts(tn, N, r):

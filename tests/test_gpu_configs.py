@@ -291,7 +291,8 @@ def kdtree_features(tn, N, idx):
     """The oracle's eight features (oracle/ldcf_oracle.py: features) of the
     samples idx of a large trajectory: the same FP32 radius and angles, the
     neighbour counts through a k-d tree over the nearest cells of all samples
-    (Chebyshev radius r) instead of the oracle's O(M^2) loop."""
+    (Chebyshev radius r) instead of the oracle's O(M^2) loop, the shell
+    density of the oracle over all samples."""
     import ldcf_oracle as lo
     from scipy.spatial import cKDTree
     dim = tn.shape[1]
@@ -320,8 +321,7 @@ def kdtree_features(tn, N, idx):
     rho_all = (radius(tn)[1] * h).astype(np.float32)
     rho = (r * h).astype(np.float32)
     f[:, 6] = np.minimum(rho_all.max() - rho, np.float32(4.0)).astype(np.float64) * 0.25
-    n0 = np.asarray(tree.query_ball_point(cells[idx], 0.5, p=np.inf, return_length=True)) - 1
-    f[:, 7] = np.log1p(n0) / 8.0
+    f[:, 7] = lo.shell_density(rho_all, N, dim)[idx]
     return f
 
 
@@ -330,7 +330,8 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     """The shipped checkpoint on c2's kooshball (32 coils): its DCF against the
     oracle's features + network on a seeded subset of samples, and the image
     against the 10-iteration Pipe-Menon image (NRMSE without a scale fit,
-    the judge of this configuration)."""
+    the judge of this configuration): at most 1.5% from the network alone,
+    at most 1% with the sampled identity scale (fr_dcf_identity_scale)."""
     import ldcf_oracle as lo
     from fastrecon_amd import ldcf
     N, image = 256, 128
@@ -347,7 +348,7 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     f = kdtree_features(tn, N, idx)
     params, mu, sigma = mlp.get_params()
     y, _ = lo.forward(params, f.astype(np.float32), emulate=True)
-    w_ref = np.exp(mu + sigma * y.astype(np.float64))
+    w_ref = lo.dcf_head(y, f, mu, sigma, mlp.base)
     dw = np.abs(w_ml[idx] / w_ref - 1.0)
     # the image judge
     rc = (image,) * 3
@@ -355,20 +356,22 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     im_it = ldcf.recon_image(ctx, p, s, N, rc)
     mlp.dcf_from_traj(s, N)
     im_ml = ldcf.recon_image(ctx, p, s, N, rc)
+    ldcf.learned_dcf(ctx, mlp, s, N, scale_plan=p)       # + the sampled identity scale
+    im_ms = ldcf.recon_image(ctx, p, s, N, rc)
     s.set_dcf(ramp)
     im_rp = ldcf.recon_image(ctx, p, s, N, rc)
-    e_ml, e_rp = ldcf.nrmse(im_ml, im_it), ldcf.nrmse(im_rp, im_it)
+    e_ml, e_ms, e_rp = ldcf.nrmse(im_ml, im_it), ldcf.nrmse(im_ms, im_it), ldcf.nrmse(im_rp, im_it)
     es_ml, es_rp = ldcf.nrmse_scaled(im_ml, im_it), ldcf.nrmse_scaled(im_rp, im_it)
     for h in (p, s):
         h.close()
     mlp.close()
     log(f"[c5] shipped network on c2 geometry: DCF vs oracle features+network (4000 samples) median "
-        f"{np.median(dw):.2e}, max {dw.max():.2e}; image NRMSE vs iterative: learned {e_ml:.4f} (scaled {es_ml:.4f}), "
-        f"ramp {e_rp:.4f} (scaled {es_rp:.4f})")
+        f"{np.median(dw):.2e}, max {dw.max():.2e}; image NRMSE vs iterative: learned {e_ml:.4f} (scaled {es_ml:.4f}, "
+        f"sampled identity scale {e_ms:.4f}), ramp {e_rp:.4f} (scaled {es_rp:.4f})")
     # BF16 hidden units: a rounding flip moves y by ~1e-3 (sigma times that in the DCF)
     assert np.median(dw) <= 1e-4 and dw.max() <= 2e-2
-    assert e_ml < e_rp
-    assert e_ml <= 0.05
+    assert e_ml < e_rp and es_ml < es_rp
+    assert e_ml <= 0.015 and e_ms <= 0.01
 
 
 def test_c5_2d_network_on_c1_geometry(ctx, orc):
@@ -394,7 +397,7 @@ def test_c5_2d_network_on_c1_geometry(ctx, orc):
     f = kdtree_features(tn, N, idx)
     params, mu, sigma = mlp.get_params()
     y, _ = lo.forward(params, f.astype(np.float32), emulate=True)
-    w_ref = np.exp(mu + sigma * y.astype(np.float64))
+    w_ref = lo.dcf_head(y, f, mu, sigma, mlp.base)
     dw = np.abs(w_ml[idx] / w_ref - 1.0)
     rc = (image, image, 1)
     hip.pipe_menon(p, s, 10, True)
@@ -421,8 +424,9 @@ def test_c5_shipped_network_phantom_and_psf(ctx):
     """The shipped checkpoint on c2's geometry against the analytic 3D
     ellipsoid phantom (single coil) and the point-spread function (all-ones
     data): NRMSE against the 10-iteration Pipe-Menon image without a scale
-    fit, and the PSF sidelobe level (max outside a 2-pixel ball over the
-    peak), each no worse than the analytic ramp DCF's."""
+    fit, below the analytic ramp DCF's with its least-squares scale; the PSF
+    sidelobe level (max outside a 2-pixel ball over the peak) no worse than
+    the ramp's."""
     from fastrecon_amd import ldcf
     N, image = 256, 128
     traj, ramp = syn.kooshball3d(51200, 256)
@@ -447,6 +451,7 @@ def test_c5_shipped_network_phantom_and_psf(ctx):
         f"identity-scaled {r['nrmse_learned_identity_scaled']:.4f}), ramp {r['nrmse_ramp']:.4f} (scaled "
         f"{r['nrmse_ramp_scaled']:.4f}); PSF sidelobe learned {side['learned']:.4f}, iterative "
         f"{side['iterative']:.4f}, ramp {side['ramp']:.4f}")
-    assert r["nrmse_learned"] <= r["nrmse_ramp"]
-    assert r["nrmse_learned"] <= 0.06
+    # the learned DCF without any scale fit against the ramp with the best one
+    assert r["nrmse_learned"] <= r["nrmse_ramp_scaled"]
+    assert r["nrmse_learned"] <= 0.025
     assert side["learned"] <= side["ramp"]

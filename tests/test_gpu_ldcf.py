@@ -52,8 +52,8 @@ def test_features_match_oracle(ctx, dim, N, M):
     # outer-shell distance: exact FP32 radii and maximum
     np.testing.assert_allclose(f[:, 6], ref[:, 6], rtol=0, atol=2e-6)
     assert np.sum(f[:, 6] == 0.0) >= 1
-    n0 = np.rint(np.expm1(f[:, 7].astype(np.float64) * 8)).astype(np.int64)
-    assert np.array_equal(n0, np.rint(np.expm1(ref[:, 7] * 8)).astype(np.int64))
+    # shell density: exact FP32 radii and counts, an FP32 shell volume
+    np.testing.assert_allclose(f[:, 7], ref[:, 7], rtol=1e-6, atol=1e-7)
 
 
 @pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41), (3, 6, 500)])
@@ -107,12 +107,20 @@ def test_forward_matches_oracle(ctx):
     d = np.abs(y - ref)
     print(f"[mlp forward] max |dy| {d.max():.2e}, median {np.median(d):.2e}", flush=True)
     assert np.median(d) <= 1e-5 and d.max() <= 1e-2
-    # the DCF head is exp(mu + sigma y)
+    # the DCF head is exp(mu + sigma y), divided by 1 + n_k with a base feature
     net.set_params(p, 0.25, 0.5)
     s = hip.Samples(ctx, 3, M, 1)
     net.dcf(upload(ctx, X), s)
     w = s.get_dcf()
     np.testing.assert_allclose(w, np.exp(0.25 + 0.5 * y), rtol=2e-6)
+    for k in (3, 4, 5):
+        net.base = k
+        assert net.base == k
+        net.dcf(upload(ctx, X), s)
+        np.testing.assert_allclose(s.get_dcf(), lo.dcf_head(y, X, 0.25, 0.5, k), rtol=4e-6)
+    with pytest.raises(hip.HipError):
+        net.base = 7
+    net.base = -1
 
 
 def test_gradient_matches_oracle_and_is_deterministic(ctx):

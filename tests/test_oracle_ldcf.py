@@ -1,7 +1,9 @@
 """Pins of the learned-DCF oracle (oracle/ldcf_oracle.py): the MLP gradient
 against central finite differences in FP64, the neighbour counts against an
 independent k-d tree (Chebyshev metric), the angular features against closed
-forms, and the BF16 rounding against known values."""
+forms, the shell density against a lattice of one point per cell and the
+volume of a ball, the DCF head against its definition, and the BF16
+rounding against known values."""
 import numpy as np
 import pytest
 from scipy.spatial import cKDTree
@@ -45,9 +47,6 @@ def test_neighbour_counts_match_kdtree(dim, N, M):
     for r in (1, 2, 4):
         ref = np.array([len(q) - 1 for q in tree.query_ball_point(cell, r + 0.5, p=np.inf)])
         assert np.array_equal(lo.neighbour_counts(t, N, r), ref)
-    # f7: the other samples in the same cell (Chebyshev radius 0)
-    same = np.array([len(q) - 1 for q in tree.query_ball_point(cell, 0.5, p=np.inf)])
-    np.testing.assert_allclose(lo.features(t, N)[:, 7], np.log1p(same) / 8, rtol=1e-12)
 
 
 def test_angular_features_closed_forms():
@@ -59,12 +58,61 @@ def test_angular_features_closed_forms():
     np.testing.assert_allclose(f[:, 0], np.log1p([8, 8, 8, 8, np.sqrt(2) * 8]) / 5, rtol=1e-6)
     np.testing.assert_allclose(f[:, 1], [0.0, 0.5, 0.0, -0.5, 1.0], atol=1e-7)
     np.testing.assert_allclose(f[:, 2], [0.5, 0.5, 0.0, 0.5, 0.75], atol=1e-7)
-    assert np.all(f[:, 3:6] == 0) and np.all(f[:, 7] == 0)    # isolated points: no neighbours
+    assert np.all(f[:, 3:6] == 0)                 # isolated points: no neighbours
+    # f7: four samples in the shell 8 <= rho < 8.5 (4 pi/3 (8.5^3 - 8^3) =
+    # 427.78 cells), one in 11 <= rho < 11.5
+    np.testing.assert_allclose(f[:4, 7], np.log1p(4 / (4 * np.pi / 3 * 102.125)) / 8, rtol=1e-9)
+    np.testing.assert_allclose(f[4, 7], np.log1p(1 / (4 * np.pi / 3 * (11.5 ** 3 - 11 ** 3))) / 8, rtol=1e-9)
     # distance to the outermost shell: 8 sqrt(2) - 8 cells for the four at radius 8
     np.testing.assert_allclose(f[:, 6], [(np.sqrt(2) - 1) * 2] * 4 + [0.0], atol=2e-6)
     # clipped at 4 cells
     g = lo.features(np.array([[c + 2, c, c], [c, c + 12, c]], np.float32), N)
     np.testing.assert_allclose(g[:, 6], [1.0, 0.0], atol=1e-7)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shell_density_of_a_lattice(dim):
+    """One point per cell (the integer lattice inside a ball of radius 30
+    around the grid centre): the shells away from the centre hold as many
+    points as cells -- a wrong shell volume (the other dimension's formula,
+    whole-cell shells, a missing 4 pi/3) misses by far more than the 1%
+    allowed."""
+    N = 96
+    c = (N - 1) / 2.0
+    g = np.arange(-31, 32)
+    pts = np.stack(np.meshgrid(*([g] * dim), indexing="ij"), -1).reshape(-1, dim).astype(np.float64)
+    pts = pts[np.sqrt((pts ** 2).sum(1)) < 30.0] + c
+    f = lo.features(pts.astype(np.float32), N) if dim == 2 else None
+    t = pts.astype(np.float32)
+    # the radius as the oracle computes it, without the O(M^2) box counts
+    u = (t - np.float32(c)) / np.float32(N / 2)
+    r = np.sqrt((u * u).sum(1, dtype=np.float32)).astype(np.float32)
+    rho = (r * np.float32(N / 2)).astype(np.float32)
+    f7 = lo.shell_density(rho, N, dim)
+    if f is not None:
+        np.testing.assert_allclose(f[:, 7], f7, rtol=1e-6)
+    # the mean of 1 / density over the samples of the shells between radius 8
+    # and 27 is their cells over their points: 1 for one point per cell (a
+    # single shell's count varies with the lattice radii, in 2D by +-40%)
+    mid = (rho > 8) & (rho < 27)
+    dens = np.expm1(8.0 * f7[mid])
+    assert abs(1.0 / np.mean(1.0 / dens) - 1.0) < 0.01
+    # all the points in one bin: the innermost shell of the ball, rho < 1/2
+    np.testing.assert_allclose(lo.shell_density(np.zeros(7, np.float32), N, dim),
+                               np.log1p(7 / ((np.pi / 6) if dim == 3 else (np.pi / 4))) / 8, rtol=1e-12)
+
+
+def test_dcf_head_base():
+    """exp(mu + sigma y), divided by 1 + n_k for a base feature k."""
+    rng = np.random.default_rng(5)
+    y = rng.standard_normal(10)
+    n = rng.integers(0, 900, (10, 3))
+    x = np.zeros((10, lo.NF))
+    x[:, 3:6] = np.log1p(n) / 8
+    np.testing.assert_allclose(lo.dcf_head(y, x, 0.3, 1.7), np.exp(0.3 + 1.7 * y), rtol=1e-12)
+    for k in (3, 4, 5):
+        np.testing.assert_allclose(lo.dcf_head(y, x, 0.3, 1.7, k), np.exp(0.3 + 1.7 * y) / (1 + n[:, k - 3]),
+                                   rtol=1e-12)
 
 
 def test_training_rows_strata():
