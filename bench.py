@@ -416,8 +416,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes": bytes_alg,
-                     "kernel": f"gridding pass: k_pos_dcf + {kernel} + k_combine (fr_grid_execute), HIP events "
-                               "on the context stream",
+                     "kernel": f"gridding pass: {kernel} + k_combine (fr_grid_execute; k_pos_dcf only on the "
+                               "first pass, the DCF being unchanged after it), HIP events on the context stream",
                      "sources_sha16": sources_hash()},
         "plan_ms": plan_ms, "plan_taps": info["taps"], "plan_bytes": info["plan_bytes"],
     }
