@@ -227,6 +227,39 @@ def train(ctx, X, W, steps=3000, batch=65536, lr=3e-3, seed=11, log=print, log_e
     return mlp, mu, sigma, losses
 
 
+def calibrate_scale(ctx, mlp, dim=3, count=20, seed=7, scale=1.0, log=None):
+    """Least-squares global scale of the network's DCF on whole training
+    trajectories: c = sum(w_hat w) / sum(w_hat^2) over every sample of the
+    first `count` trajectories of the training set, w the 10-iteration
+    Pipe-Menon DCF, w_hat the network's.  For white k-space data the image
+    error of a DCF error dw is about sum(dw^2) over the samples, so c is the
+    scale that minimises it on that set.  The head's mu becomes mu + log c
+    (no trajectory outside the training set is used).  Returns c."""
+    gen = syn.ldcf_training_trajectories if dim == 3 else syn.ldcf_training_trajectories_2d
+    N = N_GRID if dim == 3 else N_GRID_2D
+    num = den = 0.0
+    for n, (name, k) in enumerate(gen(count, seed, scale)):
+        tn, _ = hip.normalize_traj(k, N)
+        s, p = plan_for(ctx, tn, N)
+        hip.pipe_menon(p, s, 10, True)
+        w = s.get_dcf().astype(np.float64)
+        mlp.dcf_from_traj(s, N)
+        wh = s.get_dcf().astype(np.float64)
+        ok = usable_targets(w)
+        num += float(wh[ok] @ w[ok])
+        den += float(wh[ok] @ wh[ok])
+        for h in (p, s):
+            h.close()
+        if log:
+            log(f"[ldcf calibrate] {n + 1}/{count} {name}: running scale {num / den:.5f}")
+    c = num / den
+    params, mu, sigma = mlp.get_params()
+    base = mlp.base
+    mlp.set_params(params, mu + float(np.log(c)), sigma)
+    mlp.base = base
+    return c
+
+
 def save(mlp, path=CHECKPOINT, meta=None):
     p, mu, sigma = mlp.get_params()
     np.savez(path, params=p, mu=np.float32(mu), sigma=np.float32(sigma), base=np.int32(mlp.base),
