@@ -330,8 +330,8 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     """The shipped checkpoint on c2's kooshball (32 coils): its DCF against the
     oracle's features + network on a seeded subset of samples, and the image
     against the 10-iteration Pipe-Menon image (NRMSE without a scale fit,
-    the judge of this configuration): at most 1.5% from the network alone,
-    at most 1% with the sampled identity scale (fr_dcf_identity_scale)."""
+    the judge of this configuration): at most 1% from the network alone and
+    with the sampled identity scale (fr_dcf_identity_scale)."""
     import ldcf_oracle as lo
     from fastrecon_amd import ldcf
     N, image = 256, 128
@@ -371,7 +371,7 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     # BF16 hidden units: a rounding flip moves y by ~1e-3 (sigma times that in the DCF)
     assert np.median(dw) <= 1e-4 and dw.max() <= 2e-2
     assert e_ml < e_rp and es_ml < es_rp
-    assert e_ml <= 0.015 and e_ms <= 0.01
+    assert e_ml <= 0.01 and e_ms <= 0.01
 
 
 def test_c5_2d_network_on_c1_geometry(ctx, orc):
