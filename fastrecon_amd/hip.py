@@ -521,8 +521,9 @@ def pipe_menon(plan, samples, iters=10, init_ones=True):
 
 def dcf_identity_scale(plan, samples, tile_stride=1):
     """Scale the samples' DCF in place by alpha = sum(w) / |G w|^2 (the
-    Pipe-Menon fixed-point identity; |G w|^2 from every tile_stride-th unit
-    pair when > 1); returns alpha."""
+    Pipe-Menon fixed-point identity; |G w|^2 estimated from a sample of the
+    grid with stride tile_stride when > 1, fr_dcf_identity_scale); returns
+    alpha."""
     a = C.c_float(0.0)
     plan.ctx.check(plan.ctx.L.fr_dcf_identity_scale(plan.h, samples.h, int(tile_stride), C.byref(a)),
                    "fr_dcf_identity_scale")

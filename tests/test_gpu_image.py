@@ -168,11 +168,12 @@ def test_dcf_identity_scale(ctx, orc, dim):
 
 
 def test_dcf_identity_scale_sampled(ctx):
-    """The identity scale with the heavy units' |G w|^2 exact and the light
-    units' from every k-th unit pair (a regular sub-lattice of the pairs):
-    within 0.5% of the whole-grid identity on a 3D kooshball (1-iteration and
-    converged Pipe-Menon DCFs, stride 4 and 8); a plan without unit pairs
-    takes the whole grid (alpha within FP32 summation-order noise)."""
+    """The identity scale from a sample of the grid (the light units of one
+    run of four pairs in every k, the heavy units of every k-th rank; each
+    stratum scaled by its units over its sampled units): within 0.5% of the
+    whole-grid identity on a 3D kooshball (1-iteration and converged
+    Pipe-Menon DCFs, stride 4 and 8); a plan without unit pairs takes the
+    whole grid (alpha within FP32 summation-order noise)."""
     from fastrecon_amd import ldcf
     traj, _ = syn.kooshball3d(8000, 128)
     N = 128
@@ -194,6 +195,37 @@ def test_dcf_identity_scale_sampled(ctx):
         assert a[4] != a[1] and a[8] != a[1]          # the sampled path ran
         assert abs(a_nopairs / a[1] - 1) <= 1e-5
     for h in (p1, s1, p, s):
+        h.close()
+
+
+def test_identity_scale_keeps_the_plan_current(ctx):
+    """After the identity scale (whole grid and sampled) on a position-weight
+    plan, which scales the DCF and the plan's copy of it together, the next
+    gridding pass equals a fresh plan's pass on the scaled DCF, bit for bit;
+    stride 0 is refused."""
+    tn, dcf, data, N, lut, s, p, g = setup(ctx, 3, 2000, 64, 3, 32, 8)
+    hip.pipe_menon(p, s, iters=2, init_ones=True)
+    hip.grid_execute(p, s, g)                # the plan's DCF copy is now current
+    for k in (1, 8):
+        hip.dcf_identity_scale(p, s, k)
+        hip.grid_execute(p, s, g)
+        got = g.download_all()
+        s2 = hip.Samples(ctx, 3, tn.shape[0], 3)
+        s2.set_traj(tn)
+        s2.set_dcf(s.get_dcf())
+        s2.set_coils(data)
+        p2 = hip.Plan(ctx, s2, N, 4.0, lut)
+        g2 = hip.Grids(ctx, 3, N, 3)
+        hip.grid_execute(p2, s2, g2)
+        want = g2.download_all()
+        print(f"[identity keeps plan current, stride {k}] equal {np.array_equal(got.view(np.int64), want.view(np.int64))}",
+              flush=True)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64))
+        for h in (g2, p2, s2):
+            h.close()
+    with pytest.raises(hip.HipError):
+        hip.dcf_identity_scale(p, s, 0)
+    for h in (g, p, s):
         h.close()
 
 
