@@ -330,7 +330,7 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     """The shipped checkpoint on c2's kooshball (32 coils): its DCF against the
     oracle's features + network on a seeded subset of samples, and the image
     against the 10-iteration Pipe-Menon image (NRMSE without a scale fit,
-    the judge of this configuration): at most 1% from the network alone and
+    the measure of this configuration): at most 1% from the network alone and
     with the sampled identity scale (fr_dcf_identity_scale)."""
     import ldcf_oracle as lo
     from fastrecon_amd import ldcf
@@ -350,7 +350,7 @@ def test_c5_shipped_network_on_c2_geometry(ctx, orc):
     y, _ = lo.forward(params, f.astype(np.float32), emulate=True)
     w_ref = lo.dcf_head(y, f, mu, sigma, mlp.base)
     dw = np.abs(w_ml[idx] / w_ref - 1.0)
-    # the image judge
+    # the image measure
     rc = (image,) * 3
     hip.pipe_menon(p, s, 10, True)
     im_it = ldcf.recon_image(ctx, p, s, N, rc)

@@ -1,4 +1,4 @@
-"""The image judges of the learned-DCF evaluation (fastrecon_amd/ldcf.py:
+"""The image measures of the learned-DCF evaluation (fastrecon_amd/ldcf.py:
 nrmse, nrmse_scaled, psf_sidelobe) against closed forms.
 
 - nrmse(a, b) = |a - b| / |b|: zero for a = b, |c - 1| for a = c b, and
@@ -49,7 +49,7 @@ def test_nrmse_scaled_is_the_sine_of_the_angle(theta):
     assert ldcf.nrmse_scaled(a, b) == pytest.approx(np.sin(theta), rel=1e-9, abs=1e-12)
     # invariant to the scale of the first argument
     assert ldcf.nrmse_scaled(0.01 * a, b) == pytest.approx(ldcf.nrmse_scaled(a, b), rel=1e-9, abs=1e-12)
-    # never above the unscaled judge
+    # never above the unscaled measure
     assert ldcf.nrmse_scaled(a, b) <= ldcf.nrmse(a, b) + 1e-12
 
 
