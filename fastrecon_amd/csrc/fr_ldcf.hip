@@ -47,7 +47,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <initializer_list>
 
 namespace {
 
@@ -185,6 +187,219 @@ __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ tra
     if (lds_shells)
         for (int i = threadIdx.x; i < nb; i += blockDim.x)
             if (shells[i]) atomicAdd(&gshell[i], shells[i]);
+}
+
+// Binned histogram (grids of at most 8192 tiles of T^dim cells): the same
+// counts without one global atomic per sample.  k_bin_count counts samples
+// per tile (and shells, rho_max) in LDS, adds the tile totals once per
+// workgroup and keeps each sample's key (tile << 16 | cell within the tile);
+// k_bin_scan turns the totals into bucket offsets and the tiles' chunk
+// lists; k_bin_scatter writes each sample's cell within its tile (16 bits)
+// into its tile's bucket, slots reserved per workgroup; k_bin_hist counts
+// one chunk of BIN_CHUNK bucket entries in LDS: the first chunk of every
+// tile writes the tile's cells, a second launch adds the further chunks of
+// the dense tiles (the few around the k-space centre) with atomics.  Integer
+// counts: the order within a bucket does not matter.
+constexpr uint32_t BIN_CHUNK = 8192;
+struct bins_view {
+    int tb, nt;            // tile side 2^tb cells; nt tiles per axis
+    int64_t NT;            // tiles
+    uint32_t *total, *start, *cursor, *more;   // [NT], [NT + 1], [NT], [NT + 1]
+    uint32_t *key;         // [M]
+    uint16_t *bucket;      // [M]
+};
+
+__device__ inline void cell_of(const float *__restrict__ traj, int64_t M, int dim, int N, int64_t s, int (&c)[3])
+{
+    c[0] = c[1] = c[2] = 0;
+    for (int d = 0; d < dim; ++d) c[d] = min(max((int)floorf(traj[d * M + s] + 0.5f), 0), N - 1);
+}
+
+__device__ inline uint32_t key_of(const int (&c)[3], int dim, int tb, int nt)
+{
+    const int m = (1 << tb) - 1;
+    const uint32_t tile = dim == 3 ? (uint32_t)(((c[2] >> tb) * nt + (c[1] >> tb)) * nt + (c[0] >> tb))
+                                   : (uint32_t)((c[1] >> tb) * nt + (c[0] >> tb));
+    const uint32_t in = dim == 3 ? (uint32_t)(((c[2] & m) << (2 * tb)) | ((c[1] & m) << tb) | (c[0] & m))
+                                 : (uint32_t)(((c[1] & m) << tb) | (c[0] & m));
+    return tile << 16 | in;
+}
+
+// One LDS atomic per run of equal tiles in consecutive lanes (consecutive
+// samples of a spoke share a tile for about T samples): every lane of the
+// wave calls this together; a lane with tile < 0 adds nothing and ends the
+// run before it.  Returns the counter's value before this lane's unit.
+__device__ inline uint32_t wave_run_add(uint32_t *ctr, int tile)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(tile, 1);
+    const uint64_t bounds = __ballot(lane == 0 || prev != tile);
+    const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1;   // lanes 0..lane
+    const int head = 63 - __clzll(bounds & upto);
+    const uint64_t after = bounds & ~upto;
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    uint32_t base = 0;
+    if (lane == head && tile >= 0) base = atomicAdd(&ctr[tile], (uint32_t)(end - lane));
+    return __shfl(base, head) + (uint32_t)(lane - head);
+}
+
+// per-tile sample counts into B.total, each sample's key, shells and
+// rho_max into hist's tail (as k_cell_hist); dynamic LDS: NT tile counters,
+// then nb shell counters when lds_shells.  The loop runs whole waves (the
+// stride is a multiple of 64) so the lanes of a wave meet in wave_run_add.
+__global__ void __launch_bounds__(256) k_bin_count(const float *__restrict__ traj, int64_t M, int dim, int N,
+                                                   bins_view B, uint32_t *__restrict__ hist, int nb, int lds_shells)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t *tcnt = lds, *shells = lds + B.NT;
+    for (int64_t i = threadIdx.x; i < B.NT; i += blockDim.x) tcnt[i] = 0u;
+    if (lds_shells)
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) shells[i] = 0u;
+    __syncthreads();
+    const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
+    uint32_t *gshell = hist + ncells + 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    float rmax = 0.f;
+    for (int64_t s0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); s0 < M; s0 += stride) {
+        const int64_t s = s0 + (threadIdx.x & 63);
+        int tile = -1;
+        if (s < M) {
+            const float rho = sample_rho(traj, M, dim, N, s);
+            rmax = fmaxf(rmax, rho);
+            const int b = shell_of(rho, nb);
+            if (lds_shells)
+                atomicAdd(&shells[b], 1u);
+            else
+                atomicAdd(&gshell[b], 1u);
+            int c[3];
+            cell_of(traj, M, dim, N, s, c);
+            const uint32_t k = key_of(c, dim, B.tb, B.nt);
+            B.key[s] = k;
+            tile = (int)(k >> 16);
+        }
+        wave_run_add(tcnt, tile);
+    }
+    for (int m = 32; m >= 1; m >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, m));
+    if ((threadIdx.x & 63) == 0) atomicMax(&hist[ncells], __float_as_uint(rmax));
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < B.NT; i += blockDim.x)
+        if (tcnt[i]) atomicAdd(&B.total[i], tcnt[i]);
+    if (lds_shells)
+        for (int i = threadIdx.x; i < nb; i += blockDim.x)
+            if (shells[i]) atomicAdd(&gshell[i], shells[i]);
+}
+
+// exclusive scans of the NT (<= 8192) tile totals into start[0..NT] (and
+// cursor[0..NT)) and of the tiles' chunks beyond the first into
+// more[0..NT]; one 256-thread workgroup, 32 tiles per thread at most
+__global__ void __launch_bounds__(256) k_bin_scan(bins_view B)
+{
+    __shared__ uint32_t part[256], pmore[256];
+    const int64_t per = (B.NT + 255) / 256;
+    const int64_t i0 = threadIdx.x * per, i1 = min(i0 + per, B.NT);
+    uint32_t sum = 0, extra = 0;
+    for (int64_t i = i0; i < i1; ++i) {
+        const uint32_t t = B.total[i];
+        sum += t;
+        extra += t > BIN_CHUNK ? (t - 1) / BIN_CHUNK : 0u;
+    }
+    part[threadIdx.x] = sum;
+    pmore[threadIdx.x] = extra;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0, runm = 0;
+        for (int t = 0; t < 256; ++t) {
+            const uint32_t v = part[t], w = pmore[t];
+            part[t] = run;
+            pmore[t] = runm;
+            run += v;
+            runm += w;
+        }
+        B.start[B.NT] = run;
+        B.more[B.NT] = runm;
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x], runm = pmore[threadIdx.x];
+    for (int64_t i = i0; i < i1; ++i) {
+        const uint32_t t = B.total[i];
+        B.start[i] = run;
+        B.cursor[i] = run;
+        B.more[i] = runm;
+        run += t;
+        runm += t > BIN_CHUNK ? (t - 1) / BIN_CHUNK : 0u;
+    }
+}
+
+// each sample's cell within its tile into the tile's bucket: the workgroup
+// counts its samples per tile, reserves that many slots per tile at once,
+// then hands them out in LDS (whole waves per step, as k_bin_count)
+__global__ void __launch_bounds__(256) k_bin_scatter(int64_t M, bins_view B)
+{
+    extern __shared__ uint32_t tcnt[];
+    for (int64_t i = threadIdx.x; i < B.NT; i += blockDim.x) tcnt[i] = 0u;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    for (int64_t s0 = first; s0 < M; s0 += stride) {
+        const int64_t s = s0 + (threadIdx.x & 63);
+        wave_run_add(tcnt, s < M ? (int)(B.key[s] >> 16) : -1);
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < B.NT; i += blockDim.x)
+        if (tcnt[i]) tcnt[i] = atomicAdd(&B.cursor[i], tcnt[i]);
+    __syncthreads();
+    for (int64_t s0 = first; s0 < M; s0 += stride) {
+        const int64_t s = s0 + (threadIdx.x & 63);
+        const uint32_t k = s < M ? B.key[s] : ~0u;
+        const uint32_t slot = wave_run_add(tcnt, s < M ? (int)(k >> 16) : -1);
+        if (s < M) B.bucket[slot] = (uint16_t)(k & 0xffffu);
+    }
+}
+
+// one chunk of a tile's bucket counted in LDS (T^dim counters, dynamic).
+// extra == 0: block b takes the first BIN_CHUNK entries of tile b and
+// writes every cell of the tile inside the grid; extra == 1: block b takes
+// the b-th chunk beyond the first (tile found in B.more) and adds its
+// non-zero counts to cells the first launch wrote
+__global__ void __launch_bounds__(256) k_bin_hist(bins_view B, int dim, int N, uint32_t *__restrict__ hist, int extra)
+{
+    extern __shared__ uint32_t h[];
+    int64_t tile = blockIdx.x;
+    uint32_t j = 0;
+    if (extra) {
+        const uint32_t item = blockIdx.x;
+        if (item >= B.more[B.NT]) return;
+        // the last tile t with more[t] <= item (more is non-decreasing and
+        // more[NT] > item, so more[t + 1] > item: t has the item)
+        int64_t lo = 0, hi = B.NT;   // more[lo] <= item < more[hi]
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (B.more[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        tile = lo;
+        j = item - B.more[lo] + 1;
+    }
+    const int tb = B.tb, T = 1 << tb;
+    const int cellsT = dim == 3 ? T * T * T : T * T;
+    for (int i = threadIdx.x; i < cellsT; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const uint32_t end = B.start[tile + 1];
+    const uint32_t e0 = B.start[tile] + j * BIN_CHUNK;
+    const uint32_t e1 = min(end, e0 + BIN_CHUNK);
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) atomicAdd(&h[B.bucket[e]], 1u);
+    __syncthreads();
+    const int tx = (int)(tile % B.nt), ty = (int)((tile / B.nt) % B.nt), tz = dim == 3 ? (int)(tile / ((int64_t)B.nt * B.nt)) : 0;
+    for (int i = threadIdx.x; i < cellsT; i += blockDim.x) {
+        const int x = (tx << tb) + (i & (T - 1)), y = (ty << tb) + ((i >> tb) & (T - 1)), z = (tz << tb) + (i >> (2 * tb));
+        if (x < N && y < N && (dim == 2 || z < N)) {
+            const size_t cell = ((size_t)(dim == 3 ? z : 0) * N + y) * N + x;
+            if (!extra)
+                hist[cell] = h[i];
+            else if (h[i])
+                atomicAdd(&hist[cell], h[i]);
+        }
+    }
 }
 
 // inclusive prefix sum along one axis; axis stride `st`, lines enumerated by
@@ -813,22 +1028,65 @@ extern "C" int fr_mlp_param_count(void) { return NPARAM; }
 
 // The count tables of a trajectory in one buffer: the histogram /
 // summed-area table (cells uint32), rho_max, the shell counts (nb uint32;
-// radii up to N/2 sqrt(dim) in half cells, plus margin), zeroed before each
-// use (zero_bytes); then from the next 16-byte boundary the cells' box
-// counts (uint4 each).
+// radii up to N/2 sqrt(dim) in half cells, plus margin); then from the next
+// 16-byte boundary the cells' box counts (uint4 each); then, for a binned
+// histogram, the tile totals, starts, cursors and chunk offsets, the M
+// sample keys and the M-entry bucket.
+// Zeroed before each use: everything up to the box counts (zero_bytes), or,
+// binned, rho_max, the shells and the tile totals (the tiles write every
+// cell).
+constexpr int64_t MAX_TILES = 8192;
 struct counts_layout {
-    size_t cells, zero_bytes, box_off, bytes;
+    size_t cells, zero_bytes, box_off, bins_off, bytes;
     int nb;
+    bool binned;
+    int tb, nt;            // binned: tile side 2^tb, nt tiles per axis
+    int64_t NT, M;
 };
-static counts_layout layout_of(int dim, unsigned N)
+static counts_layout layout_of(int dim, unsigned N, int64_t M)
 {
     counts_layout L;
     L.cells = fr_cells(dim, N);
     L.nb = (int)std::ceil((double)N * std::sqrt((double)dim)) + 2;
     L.zero_bytes = (L.cells + 1 + (size_t)L.nb) * sizeof(uint32_t);
     L.box_off = (L.zero_bytes + 15) & ~(size_t)15;
-    L.bytes = L.box_off + L.cells * sizeof(uint4);
+    L.bins_off = L.box_off + L.cells * sizeof(uint4);
+    // tiles of 16^3 cells in 3D, 64^2 or 128^2 in 2D (at most 64 KB of LDS
+    // counters per tile), at most MAX_TILES of them; bucket slots are 32-bit
+    L.binned = false;
+    L.tb = L.nt = 0;
+    L.NT = 0;
+    L.M = M;
+    for (int tb : dim == 3 ? std::initializer_list<int>{4} : std::initializer_list<int>{6, 7}) {
+        if (M <= 0 || M > (int64_t)UINT32_MAX) break;
+        const int nt = (int)((N + (1u << tb) - 1) >> tb);
+        const int64_t NT = dim == 3 ? (int64_t)nt * nt * nt : (int64_t)nt * nt;
+        if (NT <= MAX_TILES) {
+            L.binned = true;
+            L.tb = tb;
+            L.nt = nt;
+            L.NT = NT;
+            break;
+        }
+    }
+    L.bytes = L.bins_off +
+              (L.binned ? ((4 * (size_t)L.NT + 2 + (size_t)M) * sizeof(uint32_t) + (size_t)M * sizeof(uint16_t)) : 0);
     return L;
+}
+static bins_view bins_of(uint32_t *counts, const counts_layout &L)
+{
+    uint32_t *t = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(counts) + L.bins_off);
+    uint32_t *key = t + 4 * L.NT + 2;
+    return bins_view{L.tb, L.nt, L.NT, t, t + L.NT, t + 2 * L.NT + 1, t + 3 * L.NT + 1, key,
+                     reinterpret_cast<uint16_t *>(key + L.M)};
+}
+// zero what the tables' passes accumulate into
+static hipError_t zero_counts(hipStream_t q, uint32_t *counts, const counts_layout &L)
+{
+    if (!L.binned) return hipMemsetAsync(counts, 0, L.zero_bytes, q);
+    hipError_t e = hipMemsetAsync(counts + L.cells, 0, (1 + (size_t)L.nb) * sizeof(uint32_t), q);
+    if (e == hipSuccess) e = hipMemsetAsync(bins_of(counts, L).total, 0, (size_t)L.NT * sizeof(uint32_t), q);
+    return e;
 }
 static counts_view view_of(const uint32_t *counts, const counts_layout &L)
 {
@@ -847,8 +1105,23 @@ static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32
     // box is flushed a few thousand times, not once per spoke
     const unsigned wgs = std::max(1u, std::min(nblk(s->M, 256), nblk(s->M, 256 * 32)));
     const int lds = L.nb <= LDS_SHELLS;
-    hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), lds ? sizeof(uint32_t) * L.nb : 0, q, s->traj, s->M, s->dim,
-                       (int)N, counts, L.nb, lds);
+    if (L.binned) {
+        const bins_view B = bins_of(counts, L);
+        const size_t tile_lds = sizeof(uint32_t) * (size_t)L.NT;
+        const size_t cell_lds = sizeof(uint32_t) << (s->dim * L.tb);
+        hipLaunchKernelGGL(k_bin_count, dim3(wgs), dim3(256), tile_lds + (lds ? sizeof(uint32_t) * L.nb : 0), q,
+                           s->traj, s->M, s->dim, (int)N, B, counts, L.nb, lds);
+        hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, q, B);
+        hipLaunchKernelGGL(k_bin_scatter, dim3(wgs), dim3(256), tile_lds, q, s->M, B);
+        hipLaunchKernelGGL(k_bin_hist, dim3((unsigned)L.NT), dim3(256), cell_lds, q, B, s->dim, (int)N, counts, 0);
+        // chunks beyond the tiles' first: at most M / BIN_CHUNK
+        const unsigned more = (unsigned)(s->M / BIN_CHUNK);
+        if (more > 0)
+            hipLaunchKernelGGL(k_bin_hist, dim3(more), dim3(256), cell_lds, q, B, s->dim, (int)N, counts, 1);
+    } else {
+        hipLaunchKernelGGL(k_cell_hist, dim3(wgs), dim3(256), lds ? sizeof(uint32_t) * L.nb : 0, q, s->traj, s->M,
+                           s->dim, (int)N, counts, L.nb, lds);
+    }
     const int64_t lines = s->dim == 3 ? (int64_t)N * N : N;
     hipLaunchKernelGGL(k_prefix_x, dim3(nblk(lines, 4)), dim3(256), 0, q, counts, (int)N, lines);
     for (int ax = 1; ax < s->dim; ++ax)
@@ -867,12 +1140,12 @@ extern "C" fr_status fr_ldcf_features(fr_ctx *ctx, const fr_samples *s, unsigned
     FR_CHECK(ctx, fr_cells(s->dim, N) < (size_t(1) << 31), FR_ERR_UNSUPPORTED,
              "learned-DCF features on %u^%d cells (at most 2^31 cells)", N, s->dim);
     FR_EXTENT(ctx, feat_dev, sizeof(float) * FSTRIDE * (size_t)s->M, "fr_ldcf_features rows");
-    const counts_layout L = layout_of(s->dim, N);
+    const counts_layout L = layout_of(s->dim, N, s->M);
     uint32_t *hist = nullptr;
     fr_status st = fr_alloc(ctx, (void **)&hist, L.bytes);
     if (st) return st;
     hipStream_t q = ctx->stream;
-    hipError_t e = hipMemsetAsync(hist, 0, L.zero_bytes, q);
+    hipError_t e = zero_counts(q, hist, L);
     if (e == hipSuccess && s->M > 0) {
         launch_counts(q, s, N, hist, L);
         hipLaunchKernelGGL(k_features, dim3(nblk(s->M, 256)), dim3(256), 0, q, s->traj, s->M, s->dim, (int)N,
@@ -985,7 +1258,7 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
     FR_CHECK(ctx, N >= 2 && (s->dim == 2 || s->dim == 3), FR_ERR_INVALID, "bad grid size %u / dim %d", N, s->dim);
     FR_CHECK(ctx, fr_cells(s->dim, N) < (size_t(1) << 31), FR_ERR_UNSUPPORTED,
              "learned-DCF features on %u^%d cells (at most 2^31 cells)", N, s->dim);
-    const counts_layout L = layout_of(s->dim, N);
+    const counts_layout L = layout_of(s->dim, N, s->M);
     if (L.bytes > m->hist_bytes) {
         FR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         hipFree(m->hist);
@@ -996,7 +1269,7 @@ extern "C" fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N)
         m->hist_bytes = L.bytes;
     }
     if (s->M > 0) {
-        FR_HIP(ctx, hipMemsetAsync(m->hist, 0, L.zero_bytes, ctx->stream));
+        FR_HIP(ctx, zero_counts(ctx->stream, m->hist, L));
         launch_counts(ctx->stream, s, N, m->hist, L);
         const int64_t groups = (s->M + 63) / 64;
         const int64_t waves = std::min<int64_t>(groups, (int64_t)std::max(ctx->cu_count, 1) * 32);

@@ -63,12 +63,7 @@ def features(tn, N):
         f[:, 2] = np.where(r > 0, np.arccos(c) / np.pi, 0.5)
     else:
         f[:, 2] = 0.5
-    cell = np.clip(np.floor(t + np.float32(0.5)), 0, N - 1).astype(np.int64)
-    for q, rr in enumerate((1, 2, 4)):
-        n = np.zeros(M, np.int64)
-        for i in range(M):
-            n[i] = np.count_nonzero(np.max(np.abs(cell - cell[i]), axis=1) <= rr) - 1
-        f[:, 3 + q] = np.log1p(n) / 8.0
+    f[:, 3:6] = np.log1p(neighbour_counts(t, N, (1, 2, 4))) / 8.0
     rho = (r * h).astype(np.float32)
     f[:, 6] = np.minimum((rho.max() - rho).astype(np.float32), np.float32(4.0)).astype(np.float64) * 0.25
     f[:, 7] = shell_density(rho, N, d)
@@ -103,9 +98,20 @@ def dcf_head(y, x, mu, sigma, base=-1):
 
 
 def neighbour_counts(tn, N, r):
+    """Other samples whose nearest cell lies within Chebyshev distance r of
+    this sample's: every pair compared (blocks of 128 rows at a time).  r is
+    one radius (an (M,) result) or a tuple of radii (an (M, len(r)) one)."""
     t = np.asarray(tn, np.float32)
-    cell = np.clip(np.floor(t + np.float32(0.5)), 0, N - 1).astype(np.int64)
-    return np.array([np.count_nonzero(np.max(np.abs(cell - c), axis=1) <= r) - 1 for c in cell])
+    radii = np.atleast_1d(np.asarray(r, np.int64))
+    cell = np.clip(np.floor(t + np.float32(0.5)), 0, N - 1).astype(np.int32)
+    n = np.empty((len(cell), len(radii)), np.int64)
+    for i in range(0, len(cell), 128):
+        d = np.abs(cell[i:i + 128, None, 0] - cell[None, :, 0])
+        for k in range(1, cell.shape[1]):
+            np.maximum(d, np.abs(cell[i:i + 128, None, k] - cell[None, :, k]), out=d)
+        for q, rr in enumerate(radii):
+            n[i:i + 128, q] = np.count_nonzero(d <= rr, axis=1) - 1
+    return n if np.ndim(r) else n[:, 0]
 
 
 def silu(z):

@@ -32,15 +32,21 @@ def samples_for(ctx, tn):
     return s
 
 
-@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (2, 48, 2500), (3, 5, 400), (2, 6, 300), (2, 6000, 2000)])
-def test_features_match_oracle(ctx, dim, N, M):
-    """Half the points gathered around the grid centre (the cells every
-    radial spoke crosses, counted per workgroup in LDS), half uniform; N = 5
-    and 6 are smaller than that central box; N = 6000 in 2D has more radius
-    shells (8487) than the histogram pass counts in LDS, so it counts them
-    in memory."""
+@pytest.mark.parametrize("dim,N,M,near", [(3, 32, 3000, 0.5), (2, 48, 2500, 0.5), (3, 5, 400, 0.5),
+                                          (2, 6, 300, 0.5), (2, 6000, 2000, 0.5), (3, 330, 1500, 0.5),
+                                          (3, 40, 12000, 0.95)])
+def test_features_match_oracle(ctx, dim, N, M, near):
+    """A fraction `near` of the points gathered around the grid centre, the
+    rest uniform.  Grids of at most 8192 tiles (16^3 cells in 3D, 64^2 or
+    128^2 in 2D) count cells per tile in LDS; N = 5 and 6 are smaller than
+    one tile; N = 6000 in 2D uses 128^2 tiles and has more radius shells
+    (8487) than the count pass keeps in LDS, so it counts them in memory;
+    N = 330 in 3D has 9261 tiles and counts cells with one atomic per sample
+    (a central box in LDS); N = 40 with 12000 points puts more than 8192 of
+    them in the central tile, which is counted in two chunks."""
     rng = np.random.default_rng(dim + 10)
-    t = np.concatenate([rng.normal(N / 2, 2.5, (M // 2, dim)), rng.uniform(-0.7, N - 0.3, (M - M // 2, dim))])
+    Mn = int(M * near)
+    t = np.concatenate([rng.normal(N / 2, 2.5, (Mn, dim)), rng.uniform(-0.7, N - 0.3, (M - Mn, dim))])
     t = t.astype(np.float32)
     s = samples_for(ctx, t)
     f = hip.ldcf_features(ctx, s, N).to_host(np.float32).reshape(M, 8)
@@ -58,11 +64,13 @@ def test_features_match_oracle(ctx, dim, N, M):
     np.testing.assert_allclose(f[:, 7], ref[:, 7], rtol=1e-6, atol=1e-7)
 
 
-@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41), (3, 6, 500)])
+@pytest.mark.parametrize("dim,N,M", [(3, 32, 3000), (3, 40, 37), (2, 48, 2561), (3, 64, 64 * 41), (3, 6, 500),
+                                     (3, 330, 700)])
 def test_fused_dcf_equals_features_then_network(ctx, dim, N, M):
     """fr_ldcf_dcf (features in LDS, no feature rows) is bitwise the DCF of
     fr_ldcf_features + fr_mlp_dcf; M below, at and off multiples of 64, and a
-    second call on a smaller grid reuses the kept workspace."""
+    second call on a smaller grid reuses the kept workspace (N = 330 counts
+    cells per sample, its second grid of 168 per tile)."""
     rng = np.random.default_rng(dim * 100 + M)
     t = np.concatenate([rng.normal(N / 2, 2.5, (M // 2, dim)), rng.uniform(-0.7, N - 0.3, (M - M // 2, dim))])
     t = t.astype(np.float32)
