@@ -113,12 +113,43 @@ __device__ __forceinline__ uint32_t sample_boxes(const float *__restrict__ traj,
     return n;
 }
 
-__global__ void k_box_count(const float *__restrict__ traj, int64_t M, pgeom g, uint32_t *__restrict__ cnt,
-                            uint32_t *__restrict__ nrec)
+// Per-box record counts, first added up per workgroup in an LDS hash table
+// keyed by box (consecutive samples of a spoke share most of their boxes: on
+// the 256^3 kooshball a workgroup's 256 samples make about 2000 hits on
+// about 500 distinct boxes), then flushed with one global atomic per
+// distinct box.  A hit that finds
+// no free or matching slot within BOX_PROBES probes goes to the global
+// counter directly, so every hit is counted once whatever the table holds.
+constexpr int BOX_HASH_BITS = 12;
+constexpr uint32_t BOX_HASH = 1u << BOX_HASH_BITS, BOX_PROBES = 16, BOX_EMPTY = 0xffffffffu;
+__global__ void __launch_bounds__(256) k_box_count(const float *__restrict__ traj, int64_t M, pgeom g,
+                                                   uint32_t *__restrict__ cnt, uint32_t *__restrict__ nrec)
 {
+    __shared__ uint32_t hkey[BOX_HASH], hcnt[BOX_HASH];
+    for (uint32_t i = threadIdx.x; i < BOX_HASH; i += blockDim.x) {
+        hkey[i] = BOX_EMPTY;
+        hcnt[i] = 0u;
+    }
+    __syncthreads();
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    nrec[s] = sample_boxes(traj, M, s, g, [&](uint32_t box, uint32_t) { atomicAdd(&cnt[box], 1u); });
+    if (s < M)
+        nrec[s] = sample_boxes(traj, M, s, g, [&](uint32_t box, uint32_t) {
+            uint32_t h = (box * 2654435761u) >> (32 - BOX_HASH_BITS);
+            for (uint32_t probe = 0; probe < BOX_PROBES; ++probe) {
+                const uint32_t k = atomicCAS(&hkey[h], BOX_EMPTY, box);
+                if (k == BOX_EMPTY || k == box) {
+                    atomicAdd(&hcnt[h], 1u);
+                    return;
+                }
+                h = (h + 1u) & (BOX_HASH - 1u);
+            }
+            atomicAdd(&cnt[box], 1u);
+        });
+    __syncthreads();
+    // a slot with a count holds its box (the thread that claims a slot adds
+    // to it before the barrier)
+    for (uint32_t i = threadIdx.x; i < BOX_HASH; i += blockDim.x)
+        if (hcnt[i]) atomicAdd(&cnt[hkey[i]], hcnt[i]);
 }
 
 __device__ __forceinline__ uint64_t spread3(uint32_t v)
