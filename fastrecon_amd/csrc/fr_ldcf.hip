@@ -497,7 +497,9 @@ __device__ inline uint32_t row_box(const uint32_t *__restrict__ S, int dim, int 
 
 // per cell: the sample counts of the boxes of Chebyshev radius 1, 2 and 4
 // around it (.x, .y, .z; .w = 0), from the summed-area table S; one
-// workgroup per x-row y + N z.
+// workgroup per x-row y + N z.  A cell whose radius-1 box is empty holds no
+// sample, so no feature reads it: its record is not written (about half of
+// a kooshball's 256^3 cells, the corners outside the k-space ball).
 // Workgroups go round-robin to the eight XCDs, so workgroup b takes row
 // (b % 8) per + b / 8: each XCD walks its own run of consecutive rows, whose
 // corner rows its L2 then holds.  Unsigned sums wrap, so the grouping of the
@@ -509,9 +511,10 @@ __global__ void __launch_bounds__(256) k_box_counts(const uint32_t *__restrict__
     if (rowi >= lines) return;
     const int y = (int)(rowi % (uint32_t)N), z = (int)(rowi / (uint32_t)N);
     uint4 *out = B + (size_t)rowi * N;
-    for (int x = threadIdx.x; x < N; x += blockDim.x)
-        out[x] = make_uint4(row_box(S, dim, N, x, y, z, 1), row_box(S, dim, N, x, y, z, 2),
-                            row_box(S, dim, N, x, y, z, 4), 0u);
+    for (int x = threadIdx.x; x < N; x += blockDim.x) {
+        const uint32_t n1 = row_box(S, dim, N, x, y, z, 1);
+        if (n1 != 0u) out[x] = make_uint4(n1, row_box(S, dim, N, x, y, z, 2), row_box(S, dim, N, x, y, z, 4), 0u);
+    }
 }
 
 // The count tables of one trajectory: S the summed-area table of the cell
