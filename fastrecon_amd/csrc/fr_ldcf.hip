@@ -21,12 +21,13 @@
 //   lo^3) in 3D, pi (hi^2 - lo^2) in 2D).  A smooth density for radial-type
 //   trajectories, where the cell counts near the centre vary from cell to
 //   cell with the spoke crossings.
-// The counts come from a cell histogram, its summed-area table and a pass
+// The counts come from a cell histogram (binned by tile on grids of at most
+// 8192 tiles, else one atomic per sample), its summed-area table and a pass
 // over the cells that stores each cell's three box sums (r = 1, 2, 4; 8
 // corner reads each in 3D, neighbouring cells sharing them) in one uint4,
 // so a sample reads one 16-byte cell record whatever the density; rho_max
 // (one atomic maximum per wave) and the shell counts (per workgroup in LDS)
-// come from the histogram pass.
+// come from the histogram's first pass.
 //
 // The DCF head: w = exp(mu + sigma y) for the network output y, divided by
 // 1 + n_k for a count feature k = 3..5 when the network carries that base
@@ -100,7 +101,8 @@ __device__ inline float dsilu(float z)
 }
 
 // ------------------------------------------------------------- features
-// Cell histogram of the samples' nearest cells.  Radial-type trajectories
+// Cell histogram of the samples' nearest cells, for grids too large for the
+// binned histogram below (more than 8192 tiles).  Radial-type trajectories
 // cross the k-space centre once per spoke, so the few cells around N/2 take
 // thousands of hits that serialise as same-address atomics; each workgroup
 // counts the box of HR cells around the centre in LDS over its samples
