@@ -202,11 +202,17 @@ __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ tra
 // tile writes the tile's cells, a second launch adds the further chunks of
 // the dense tiles (the few around the k-space centre) with atomics.  Integer
 // counts: the order within a bucket does not matter.
+// Tile totals and bucket cursors are kept in BIN_SETS copies, workgroup b
+// using copy b % BIN_SETS (the XCD it runs on): every workgroup touches the
+// tiles its spokes cross, about 750 at c2, so one copy would take some 300
+// atomics per address from all XCDs; the scan gives each copy its own run
+// of the tile's bucket.
 constexpr uint32_t BIN_CHUNK = 8192;
+constexpr int BIN_SETS = 8;
 struct bins_view {
     int tb, nt;            // tile side 2^tb cells; nt tiles per axis
     int64_t NT;            // tiles
-    uint32_t *total, *start, *cursor, *more;   // [NT], [NT + 1], [NT], [NT + 1]
+    uint32_t *total, *start, *cursor, *more;   // [SETS][NT], [NT + 1], [SETS][NT], [NT + 1]
     uint32_t *key;         // [M]
     uint16_t *bucket;      // [M]
 };
@@ -284,49 +290,64 @@ __global__ void __launch_bounds__(256) k_bin_count(const float *__restrict__ tra
     for (int m = 32; m >= 1; m >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, m));
     if ((threadIdx.x & 63) == 0) atomicMax(&hist[ncells], __float_as_uint(rmax));
     __syncthreads();
+    uint32_t *total = B.total + (int64_t)(blockIdx.x % BIN_SETS) * B.NT;
     for (int64_t i = threadIdx.x; i < B.NT; i += blockDim.x)
-        if (tcnt[i]) atomicAdd(&B.total[i], tcnt[i]);
+        if (tcnt[i]) atomicAdd(&total[i], tcnt[i]);
     if (lds_shells)
         for (int i = threadIdx.x; i < nb; i += blockDim.x)
             if (shells[i]) atomicAdd(&gshell[i], shells[i]);
 }
 
-// exclusive scans of the NT (<= 8192) tile totals into start[0..NT] (and
-// cursor[0..NT)) and of the tiles' chunks beyond the first into
-// more[0..NT]; one 256-thread workgroup, 32 tiles per thread at most
-__global__ void __launch_bounds__(256) k_bin_scan(bins_view B)
+// exclusive scans of the NT (<= 8192) tile totals (summed over the copies)
+// into start[0..NT] and of the tiles' chunks beyond the first into
+// more[0..NT]; cursor[c][t] = start[t] + the totals of copies before c.
+// One 1024-thread workgroup, 8 tiles per thread at most; each wave scans its
+// lanes' sums with shuffles, then the 16 wave totals are scanned in LDS.
+__global__ void __launch_bounds__(1024) k_bin_scan(bins_view B)
 {
-    __shared__ uint32_t part[256], pmore[256];
-    const int64_t per = (B.NT + 255) / 256;
-    const int64_t i0 = threadIdx.x * per, i1 = min(i0 + per, B.NT);
+    __shared__ uint32_t wsum[16], wmore[16];
+    const int64_t per = (B.NT + 1023) / 1024;
+    const int64_t i0 = min((int64_t)threadIdx.x * per, B.NT), i1 = min(i0 + per, B.NT);
     uint32_t sum = 0, extra = 0;
     for (int64_t i = i0; i < i1; ++i) {
-        const uint32_t t = B.total[i];
+        uint32_t t = 0;
+        for (int c = 0; c < BIN_SETS; ++c) t += B.total[c * B.NT + i];
         sum += t;
         extra += t > BIN_CHUNK ? (t - 1) / BIN_CHUNK : 0u;
     }
-    part[threadIdx.x] = sum;
-    pmore[threadIdx.x] = extra;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0, runm = 0;
-        for (int t = 0; t < 256; ++t) {
-            const uint32_t v = part[t], w = pmore[t];
-            part[t] = run;
-            pmore[t] = runm;
-            run += v;
-            runm += w;
+    // inclusive wave scans
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t is = sum, im = extra;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(is, d), v = __shfl_up(im, d);
+        if (lane >= d) {
+            is += u;
+            im += v;
         }
-        B.start[B.NT] = run;
-        B.more[B.NT] = runm;
+    }
+    if (lane == 63) {
+        wsum[wave] = is;
+        wmore[wave] = im;
     }
     __syncthreads();
-    uint32_t run = part[threadIdx.x], runm = pmore[threadIdx.x];
+    uint32_t run = is - sum, runm = im - extra;     // exclusive within the wave
+    for (int w = 0; w < wave; ++w) {
+        run += wsum[w];
+        runm += wmore[w];
+    }
+    if (threadIdx.x == blockDim.x - 1) {
+        B.start[B.NT] = run + sum;
+        B.more[B.NT] = runm + extra;
+    }
     for (int64_t i = i0; i < i1; ++i) {
-        const uint32_t t = B.total[i];
         B.start[i] = run;
-        B.cursor[i] = run;
         B.more[i] = runm;
+        uint32_t t = 0;
+        for (int c = 0; c < BIN_SETS; ++c) {
+            const uint32_t v = B.total[c * B.NT + i];
+            B.cursor[c * B.NT + i] = run + t;
+            t += v;
+        }
         run += t;
         runm += t > BIN_CHUNK ? (t - 1) / BIN_CHUNK : 0u;
     }
@@ -347,8 +368,11 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int64_t M, bins_view B)
         wave_run_add(tcnt, s < M ? (int)(B.key[s] >> 16) : -1);
     }
     __syncthreads();
+    // this workgroup's copy: the one k_bin_count added its totals to (same
+    // grid, same samples per workgroup)
+    uint32_t *cursor = B.cursor + (int64_t)(blockIdx.x % BIN_SETS) * B.NT;
     for (int64_t i = threadIdx.x; i < B.NT; i += blockDim.x)
-        if (tcnt[i]) tcnt[i] = atomicAdd(&B.cursor[i], tcnt[i]);
+        if (tcnt[i]) tcnt[i] = atomicAdd(&cursor[i], tcnt[i]);
     __syncthreads();
     for (int64_t s0 = first; s0 < M; s0 += stride) {
         const int64_t s = s0 + (threadIdx.x & 63);
@@ -1035,8 +1059,8 @@ extern "C" int fr_mlp_param_count(void) { return NPARAM; }
 // summed-area table (cells uint32), rho_max, the shell counts (nb uint32;
 // radii up to N/2 sqrt(dim) in half cells, plus margin); then from the next
 // 16-byte boundary the cells' box counts (uint4 each); then, for a binned
-// histogram, the tile totals, starts, cursors and chunk offsets, the M
-// sample keys and the M-entry bucket.
+// histogram, the tile totals (BIN_SETS copies), starts, cursors (BIN_SETS
+// copies) and chunk offsets, the M sample keys and the M-entry bucket.
 // Zeroed before each use: everything up to the box counts (zero_bytes), or,
 // binned, rho_max, the shells and the tile totals (the tiles write every
 // cell).
@@ -1074,23 +1098,25 @@ static counts_layout layout_of(int dim, unsigned N, int64_t M)
             break;
         }
     }
-    L.bytes = L.bins_off +
-              (L.binned ? ((4 * (size_t)L.NT + 2 + (size_t)M) * sizeof(uint32_t) + (size_t)M * sizeof(uint16_t)) : 0);
+    L.bytes = L.bins_off + (L.binned ? ((2 * (BIN_SETS + 1) * (size_t)L.NT + 2 + (size_t)M) * sizeof(uint32_t) +
+                                        (size_t)M * sizeof(uint16_t))
+                                     : 0);
     return L;
 }
 static bins_view bins_of(uint32_t *counts, const counts_layout &L)
 {
-    uint32_t *t = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(counts) + L.bins_off);
-    uint32_t *key = t + 4 * L.NT + 2;
-    return bins_view{L.tb, L.nt, L.NT, t, t + L.NT, t + 2 * L.NT + 1, t + 3 * L.NT + 1, key,
-                     reinterpret_cast<uint16_t *>(key + L.M)};
+    uint32_t *total = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(counts) + L.bins_off);
+    uint32_t *start = total + BIN_SETS * L.NT, *cursor = start + L.NT + 1;
+    uint32_t *more = cursor + BIN_SETS * L.NT, *key = more + L.NT + 1;
+    return bins_view{L.tb, L.nt, L.NT, total, start, cursor, more, key, reinterpret_cast<uint16_t *>(key + L.M)};
 }
 // zero what the tables' passes accumulate into
 static hipError_t zero_counts(hipStream_t q, uint32_t *counts, const counts_layout &L)
 {
     if (!L.binned) return hipMemsetAsync(counts, 0, L.zero_bytes, q);
     hipError_t e = hipMemsetAsync(counts + L.cells, 0, (1 + (size_t)L.nb) * sizeof(uint32_t), q);
-    if (e == hipSuccess) e = hipMemsetAsync(bins_of(counts, L).total, 0, (size_t)L.NT * sizeof(uint32_t), q);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(bins_of(counts, L).total, 0, (size_t)BIN_SETS * L.NT * sizeof(uint32_t), q);
     return e;
 }
 static counts_view view_of(const uint32_t *counts, const counts_layout &L)
@@ -1116,7 +1142,7 @@ static void launch_counts(hipStream_t q, const fr_samples *s, unsigned N, uint32
         const size_t cell_lds = sizeof(uint32_t) << (s->dim * L.tb);
         hipLaunchKernelGGL(k_bin_count, dim3(wgs), dim3(256), tile_lds + (lds ? sizeof(uint32_t) * L.nb : 0), q,
                            s->traj, s->M, s->dim, (int)N, B, counts, L.nb, lds);
-        hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, q, B);
+        hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, q, B);
         hipLaunchKernelGGL(k_bin_scatter, dim3(wgs), dim3(256), tile_lds, q, s->M, B);
         hipLaunchKernelGGL(k_bin_hist, dim3((unsigned)L.NT), dim3(256), cell_lds, q, B, s->dim, (int)N, counts, 0);
         // chunks beyond the tiles' first: at most M / BIN_CHUNK
