@@ -113,14 +113,20 @@ constexpr int HB = 2 * HR + 1;
 
 // FP32 radius in cells of the normalised point (same operations as
 // sample_features)
-__device__ inline float sample_rho(const float *__restrict__ traj, int64_t M, int dim, int N, int64_t s)
+__device__ inline float rho_of(const float (&t)[3], int dim, int N)
 {
     const float cf = 0.5f * (float)(N - 1), hN = 0.5f * (float)N;
     float u[3] = {0.f, 0.f, 0.f};
-    for (int d = 0; d < dim; ++d) u[d] = __fdiv_rn(__fsub_rn(traj[d * M + s], cf), hN);
+    for (int d = 0; d < dim; ++d) u[d] = __fdiv_rn(__fsub_rn(t[d], cf), hN);
     const float r2 = dim == 3 ? __fadd_rn(__fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1])), __fmul_rn(u[2], u[2]))
                               : __fadd_rn(__fmul_rn(u[0], u[0]), __fmul_rn(u[1], u[1]));
     return __fmul_rn(sqrtf(r2), hN);
+}
+__device__ inline float sample_rho(const float *__restrict__ traj, int64_t M, int dim, int N, int64_t s)
+{
+    float t[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < dim; ++d) t[d] = traj[d * M + s];
+    return rho_of(t, dim, N);
 }
 
 // radius shell of a sample: floor(2 rho), clamped to the nb shells
@@ -217,12 +223,6 @@ struct bins_view {
     uint16_t *bucket;      // [M]
 };
 
-__device__ inline void cell_of(const float *__restrict__ traj, int64_t M, int dim, int N, int64_t s, int (&c)[3])
-{
-    c[0] = c[1] = c[2] = 0;
-    for (int d = 0; d < dim; ++d) c[d] = min(max((int)floorf(traj[d * M + s] + 0.5f), 0), N - 1);
-}
-
 __device__ inline uint32_t key_of(const int (&c)[3], int dim, int tb, int nt)
 {
     const int m = (1 << tb) - 1;
@@ -267,20 +267,31 @@ __global__ void __launch_bounds__(256) k_bin_count(const float *__restrict__ tra
     const size_t ncells = dim == 3 ? (size_t)N * N * N : (size_t)N * N;
     uint32_t *gshell = hist + ncells + 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
     float rmax = 0.f;
-    for (int64_t s0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); s0 < M; s0 += stride) {
-        const int64_t s = s0 + (threadIdx.x & 63);
+    // the next step's coordinates are loaded before this step's counting
+    float nt[3] = {0.f, 0.f, 0.f};
+    auto load = [&](int64_t s) {
+        if (s < M)
+            for (int d = 0; d < dim; ++d) nt[d] = traj[d * M + s];
+    };
+    int64_t s0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    load(s0 + lane);
+    for (; s0 < M; s0 += stride) {
+        const int64_t s = s0 + lane;
+        const float t[3] = {nt[0], nt[1], nt[2]};
+        load(s + stride);
         int tile = -1;
         if (s < M) {
-            const float rho = sample_rho(traj, M, dim, N, s);
+            const float rho = rho_of(t, dim, N);
             rmax = fmaxf(rmax, rho);
             const int b = shell_of(rho, nb);
             if (lds_shells)
                 atomicAdd(&shells[b], 1u);
             else
                 atomicAdd(&gshell[b], 1u);
-            int c[3];
-            cell_of(traj, M, dim, N, s, c);
+            int c[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) c[d] = min(max((int)floorf(t[d] + 0.5f), 0), N - 1);
             const uint32_t k = key_of(c, dim, B.tb, B.nt);
             B.key[s] = k;
             tile = (int)(k >> 16);
