@@ -390,7 +390,10 @@ fr_status fr_mlp_forward(fr_mlp *m, const float *feat_dev, int64_t M, float *y_d
 fr_status fr_mlp_dcf(fr_mlp *m, const float *feat_dev, fr_samples *s);
 /* fr_ldcf_features + fr_mlp_dcf in one pass from the samples' trajectory on
  * an N-point grid, without feature rows in memory (bitwise the same DCF).
- * The cell-count workspace is kept in the network object; no host sync. */
+ * The cell-count workspace (20 bytes per grid cell, plus 6 per sample on
+ * grids of at most 8192 tiles, i.e. 3D up to 320^3: about 415 MB at 256^3
+ * with 13.1M samples) is kept in the network object and grows to the
+ * largest call; no host sync. */
 fr_status fr_ldcf_dcf(fr_mlp *m, fr_samples *s, unsigned N);
 /* Mean squared error of y against targ_dev over the batch rows idx_dev[B]
  * (or rows 0..B-1 if idx_dev is NULL), its gradient (host, param order) and
