@@ -210,9 +210,9 @@ __global__ void __launch_bounds__(256) k_cell_hist(const float *__restrict__ tra
 // counts: the order within a bucket does not matter.
 // Tile totals and bucket cursors are kept in BIN_SETS copies, workgroup b
 // using copy b % BIN_SETS (the XCD it runs on): every workgroup touches the
-// tiles its spokes cross, about 750 at c2, so one copy would take some 300
-// atomics per address from all XCDs; the scan gives each copy its own run
-// of the tile's bucket.
+// tiles its spokes cross, about 750 on the 256^3 kooshball of 51200 spokes,
+// so one copy would take some 300 atomics per address from all XCDs; the
+// scan gives each copy its own run of the tile's bucket.
 constexpr uint32_t BIN_CHUNK = 8192;
 constexpr int BIN_SETS = 8;
 struct bins_view {

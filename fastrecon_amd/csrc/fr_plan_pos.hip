@@ -117,9 +117,9 @@ __device__ __forceinline__ uint32_t sample_boxes(const float *__restrict__ traj,
 // keyed by box (consecutive samples of a spoke share most of their boxes: on
 // the 256^3 kooshball a workgroup's 256 samples make about 2000 hits on
 // about 500 distinct boxes), then flushed with one global atomic per
-// distinct box.  A hit that finds
-// no free or matching slot within BOX_PROBES probes goes to the global
-// counter directly, so every hit is counted once whatever the table holds.
+// distinct box.  A hit that finds no free or matching slot within
+// BOX_PROBES probes goes to the global counter directly, so every hit is
+// counted once whatever the table holds.
 constexpr int BOX_HASH_BITS = 12;
 constexpr uint32_t BOX_HASH = 1u << BOX_HASH_BITS, BOX_PROBES = 16, BOX_EMPTY = 0xffffffffu;
 __global__ void __launch_bounds__(256) k_box_count(const float *__restrict__ traj, int64_t M, pgeom g,
