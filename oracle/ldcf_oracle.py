@@ -26,9 +26,10 @@ MLP 8 -> 64 -> 64 -> 1 with SiLU, BF16 weight matrices, FP32 biases:
   z1 = x W1 + b1, h1 = bf16(silu(z1)), z2 = h1 W2 + b2, h2 = silu(z2),
   y  = h2 w3 + b3,
 with W1, W2, w3 the BF16-rounded values.  DCF head (dcf_head): w = exp(mu +
-sigma y), divided by 1 + n_k = exp(8 f_k) for a network with base feature k.  The backward restates the chain rule
-(straight through the BF16 rounding of h1) and is pinned by central finite
-differences of the same function in FP64 without rounding.
+sigma y), divided by 1 + n_k = exp(8 f_k) for a network with base feature k.
+The backward restates the chain rule (straight through the BF16 rounding of
+h1) and is pinned by central finite differences of the same function in FP64
+without rounding.
 """
 import numpy as np
 
